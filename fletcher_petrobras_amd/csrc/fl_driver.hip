@@ -1,0 +1,835 @@
+// HIP back-end of the Fletcher TTI path behind the reference driver.h API
+// (original/driver.h:8-22) plus the fl_hip.h extensions.  One process-wide
+// single-GPU state, like the reference back-ends' device singletons
+// (original/CUDA/cuda_driver.c:11-34); every entry point reports failures
+// through fl_last_error() instead of exit().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fl_zmarch.h"
+#include "fletcher/driver.h"
+#include "fletcher/fl_hip.h"
+#include "fletcher/fl_model.h"
+
+using namespace fl;
+
+// ------------------------------------------------------------------ errors
+static char g_err[512] = "";
+static int g_status = FL_OK;
+
+static int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    g_status = code;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(FL_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+extern "C" const char *fl_last_error(void) { return g_err; }
+extern "C" int fl_status(void) { return g_status; }
+extern "C" void fl_clear_error(void)
+{
+    g_err[0] = 0;
+    g_status = FL_OK;
+}
+
+// ------------------------------------------------------------------ helpers
+// FP64 -> BF16 with one round-to-nearest-even of the 8-bit significand (no FP32 detour).
+static uint16_t f64_to_bf16(double v)
+{
+    double r = v;
+    if (v != 0.0 && std::isfinite(v)) {
+        int e;
+        const double m = std::frexp(v, &e);  // v = m 2^e, 0.5 <= |m| < 1
+        r = std::ldexp(std::nearbyint(std::ldexp(m, 8)), e - 8);
+    }
+    const float f = (float)r;  // exact: r has at most 8 significant bits
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return (uint16_t)(u >> 16);
+}
+
+static uint16_t f64_to_f16(double v)
+{
+    _Float16 h = (_Float16)v;  // correctly rounded double -> half
+    uint16_t b;
+    memcpy(&b, &h, 2);
+    return b;
+}
+
+// ------------------------------------------------------------------ state
+struct Tiles {
+    int ty = 8, zt = 0;
+};
+
+struct Device {
+    bool init = false;
+    int sx = 0, sy = 0, sz = 0, bord = 0;
+    float dx = 0, dy = 0, dz = 0, dt = 0;
+    long long n = 0;
+    float *f[4] = {nullptr, nullptr, nullptr, nullptr};  // pp pc qp qc (current roles)
+    void *coef = nullptr;
+    int cm = CM_F32;
+    float *dvol = nullptr;  // two-pass first-derivative volumes (4n)
+    float *trace = nullptr;
+    int rec_iz = -1, rec_every = 0, rec_slots = 0;
+    int *ctr = nullptr;     // step counters for captured runs (2 ints)
+    float *src_tab = nullptr;
+    int src_cap = 0;
+    long long bytes = 0;
+};
+
+static Device D;
+static hipStream_t g_stream = nullptr;
+static int g_variant = FL_VARIANT_FUSED;
+static int g_precision = FL_PREC_FP32;
+static bool g_sync = true;
+static Tiles g_tiles;
+static bool g_timing = false;
+static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_ev_used, g_ev_free;
+static hipEvent_t g_reg0 = nullptr, g_reg1 = nullptr;
+
+static int ensure_stream()
+{
+    if (g_stream) return FL_OK;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) return fail(FL_ERR_HIP, "no HIP device visible");
+    HIPCHK(hipSetDevice(0));
+    HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    return FL_OK;
+}
+
+static int dev_alloc(void **p, size_t bytes, long long *acct)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(FL_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    *acct += (long long)bytes;
+    return FL_OK;
+}
+
+static int coef_count(int cm) { return cm == CM_F16R ? 6 : 10; }
+static int coef_elem(int cm) { return cm == CM_F32 ? 4 : 2; }
+
+extern "C" int fl_bytes_per_cell(void)
+{
+    const int cm = D.init ? D.cm : g_precision;
+    const int c = coef_count(cm) * coef_elem(cm);
+    if (g_variant == FL_VARIANT_TWOPASS) return 24 + c + 40;  // + write/read 4 volumes, re-read pc,qc
+    return 24 + c;
+}
+
+// Host precompute of one z slab [z0,z1) into device coefficient storage.
+static int upload_coefs(const float *vpz, const float *vsv, const float *eps, const float *del,
+                        const float *phi, const float *theta, float dt, int cm, void *dst,
+                        long long n, long long sxy, int sz)
+{
+    const int slab = std::max(1, (int)std::min<long long>(sz, (64ll << 20) / (sxy * 4)));
+    const int nc = coef_count(cm);
+    std::vector<float> f32;
+    std::vector<double> f64;
+    std::vector<uint16_t> h16;
+    for (int z0 = 0; z0 < sz; z0 += slab) {
+        const int z1 = std::min(sz, z0 + slab);
+        const long long off = (long long)z0 * sxy, m = (long long)(z1 - z0) * sxy;
+        if (cm == CM_F32) {
+            f32.resize(10 * m);
+            float *c[10];
+            for (int k = 0; k < 10; k++) c[k] = f32.data() + k * m;
+            fl_precompute_f32(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, c);
+            for (int k = 0; k < 10; k++)
+                HIPCHK(hipMemcpy((float *)dst + k * n + off, c[k], m * 4, hipMemcpyHostToDevice));
+        } else if (cm == CM_BF16 || cm == CM_F16) {
+            f64.resize(10 * m);
+            h16.resize(10 * m);
+            double *c[10];
+            for (int k = 0; k < 10; k++) c[k] = f64.data() + k * m;
+            fl_precompute_f64(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, c);
+            const double dt2 = (double)dt * dt;
+            for (long long i = 0; i < 10 * m; i++) {
+                const double v = (i >= 6 * m) ? f64[i] * dt2 : f64[i];
+                h16[i] = cm == CM_BF16 ? f64_to_bf16(v) : f64_to_f16(v);
+            }
+            for (int k = 0; k < 10; k++)
+                HIPCHK(hipMemcpy((uint16_t *)dst + k * n + off, h16.data() + k * m, m * 2,
+                                 hipMemcpyHostToDevice));
+        } else {
+            h16.resize(6 * m);
+            const double pi = 3.14159265358979323846;
+            for (long long i = 0; i < m; i++) {
+                h16[i] = f64_to_f16((double)vpz[off + i] * dt);
+                h16[m + i] = f64_to_f16((double)vsv[off + i] * dt);
+                h16[2 * m + i] = f64_to_f16((double)eps[off + i]);
+                h16[3 * m + i] = f64_to_f16((double)del[off + i]);
+                h16[4 * m + i] = f64_to_f16((double)theta[off + i]);
+                h16[5 * m + i] = f64_to_f16((double)phi[off + i] - pi);
+            }
+            for (int k = 0; k < nc; k++)
+                HIPCHK(hipMemcpy((uint16_t *)dst + k * n + off, h16.data() + k * m, m * 2,
+                                 hipMemcpyHostToDevice));
+        }
+    }
+    return FL_OK;
+}
+
+// ------------------------------------------------------------------ launch plumbing
+static int default_zt(int sx, int sy, int sz, int bord, int ty)
+{
+    const int ntx = (sx - 2 * bord + 63) / 64, nty = (sy - 2 * bord + ty - 1) / ty;
+    const int nzu = sz - 2 * bord;
+    const long long per_plane = (long long)ntx * nty;
+    const long long target = 2048;  // ~4 resident waves of 2 blocks per CU on 256 CUs
+    int nzc = (int)std::max<long long>(1, (target + per_plane - 1) / per_plane);
+    int zt = (nzu + nzc - 1) / nzc;
+    zt = std::max(zt, 16);
+    return std::min(zt, nzu);
+}
+
+static void fill_geom(StepArgs &a, int sx, int sy, int sz, int bord, int ty, int zt)
+{
+    a.sx = sx; a.sy = sy; a.sz = sz; a.bord = bord;
+    a.ntx = (sx - 2 * bord + 63) / 64;
+    a.nty = (sy - 2 * bord + ty - 1) / ty;
+    a.zt = zt;
+    const int nzc = (sz - 2 * bord + zt - 1) / zt;
+    a.nblk = a.ntx * a.nty * nzc;
+}
+
+template <int TY, int CM, bool TP>
+static void launch_zm(const StepArgs &a, hipStream_t s)
+{
+    const int g = (a.nblk + 7) / 8 * 8;
+    hipLaunchKernelGGL((zmarch<TY, CM, TP>), dim3(g), dim3(64, TY), 0, s, a);
+}
+
+static int launch_zmarch(const StepArgs &a, int ty, int cm, bool tp, hipStream_t s)
+{
+#define FL_ZM(TY)                                                          \
+    if (ty == TY) {                                                        \
+        if (tp) { launch_zm<TY, CM_F32, true>(a, s); return FL_OK; }       \
+        switch (cm) {                                                      \
+        case CM_F32: launch_zm<TY, CM_F32, false>(a, s); return FL_OK;     \
+        case CM_BF16: launch_zm<TY, CM_BF16, false>(a, s); return FL_OK;   \
+        case CM_F16: launch_zm<TY, CM_F16, false>(a, s); return FL_OK;     \
+        case CM_F16R: launch_zm<TY, CM_F16R, false>(a, s); return FL_OK;   \
+        }                                                                  \
+    }
+    FL_ZM(4)
+    FL_ZM(8)
+    FL_ZM(16)
+#undef FL_ZM
+    return fail(FL_ERR_ARG, "unsupported tile rows %d / coefficient mode %d", ty, cm);
+}
+
+static StepArgs base_args()
+{
+    StepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.pp = D.f[0]; a.pc = D.f[1]; a.qp = D.f[2]; a.qc = D.f[3];
+    a.coef = D.coef;
+    a.coef_stride = D.n;
+    a.dvol = D.dvol;
+    a.dxxinv = 1.0f / (D.dx * D.dx);
+    a.dyyinv = 1.0f / (D.dy * D.dy);
+    a.dzzinv = 1.0f / (D.dz * D.dz);
+    a.dxinv = 1.0f / D.dx;
+    a.dyinv = 1.0f / D.dy;
+    a.dzinv = 1.0f / D.dz;
+    a.dt = D.dt;
+    const int zt = g_tiles.zt > 0 ? std::min(g_tiles.zt, D.sz - 2 * D.bord)
+                                  : default_zt(D.sx, D.sy, D.sz, D.bord, g_tiles.ty);
+    fill_geom(a, D.sx, D.sy, D.sz, D.bord, g_tiles.ty, zt);
+    a.src_idx = -1;
+    a.trace = D.trace;
+    a.rec_iz = D.rec_iz;
+    a.rec_every = D.rec_every;
+    a.rec_slots = D.rec_slots;
+    return a;
+}
+
+static void swap_levels()
+{
+    std::swap(D.f[0], D.f[1]);
+    std::swap(D.f[2], D.f[3]);
+}
+
+static int ensure_dvol()
+{
+    if (D.dvol) return FL_OK;
+    int rc = dev_alloc((void **)&D.dvol, (size_t)4 * D.n * 4, &D.bytes);
+    if (rc) return rc;
+    HIPCHK(hipMemset(D.dvol, 0, (size_t)4 * D.n * 4));
+    return FL_OK;
+}
+
+// One step on the current levels (no swap).  it_host = step index for traces.
+static int launch_step(StepArgs a, int it_host, hipStream_t s)
+{
+    a.it_host = it_host;
+    if (g_variant == FL_VARIANT_FUSED) {
+        return launch_zmarch(a, g_tiles.ty, D.cm, false, s);
+    }
+    if (D.cm != CM_F32)
+        return fail(FL_ERR_ARG, "variant %d needs FP32 coefficients", g_variant);
+    if (g_variant == FL_VARIANT_TWOPASS) {
+        int rc = ensure_dvol();
+        if (rc) return rc;
+        a.dvol = D.dvol;
+        dim3 g1((D.sx + 63) / 64, (D.sy + 3) / 4, D.sz);
+        hipLaunchKernelGGL(der1_pass, g1, dim3(64, 4), 0, s, a.pc, a.qc, D.dvol, D.sx, D.sy, D.sz,
+                           D.bord, a.dxinv, a.dyinv);
+        return launch_zmarch(a, g_tiles.ty, CM_F32, true, s);
+    }
+    if (g_variant == FL_VARIANT_DIRECT) {
+        DirectArgs d;
+        d.pp = a.pp; d.pc = a.pc; d.qp = a.qp; d.qc = a.qc;
+        d.coef = (const float *)D.coef;
+        d.coef_stride = D.n;
+        d.dxxinv = a.dxxinv; d.dyyinv = a.dyyinv; d.dzzinv = a.dzzinv;
+        d.dxyinv = 1.0f / (D.dx * D.dy);
+        d.dxzinv = 1.0f / (D.dx * D.dz);
+        d.dyzinv = 1.0f / (D.dy * D.dz);
+        d.dt = D.dt;
+        d.sx = D.sx; d.sy = D.sy; d.sz = D.sz; d.bord = D.bord;
+        d.zt = 32;
+        d.trace = nullptr;
+        d.rec_iz = D.rec_iz;
+        if (D.trace && D.rec_every > 0 && it_host % D.rec_every == 0) {
+            const int slot = it_host / D.rec_every - 1;
+            if (slot >= 0 && slot < D.rec_slots)
+                d.trace = D.trace + (long long)slot * (D.sx - 2 * D.bord) * (D.sy - 2 * D.bord);
+        }
+        dim3 g((D.sx - 2 * D.bord + 63) / 64, (D.sy - 2 * D.bord + 3) / 4,
+               (D.sz - 2 * D.bord + d.zt - 1) / d.zt);
+        hipLaunchKernelGGL(direct_dercross, g, dim3(64, 4), 0, s, d);
+        return FL_OK;
+    }
+    return fail(FL_ERR_ARG, "unknown variant %d", g_variant);
+}
+
+static int take_events(hipEvent_t *e0, hipEvent_t *e1)
+{
+    if (!g_ev_free.empty()) {
+        *e0 = g_ev_free.back().first;
+        *e1 = g_ev_free.back().second;
+        g_ev_free.pop_back();
+        return FL_OK;
+    }
+    HIPCHK(hipEventCreate(e0));
+    HIPCHK(hipEventCreate(e1));
+    return FL_OK;
+}
+
+// ------------------------------------------------------------------ driver.h
+static void free_device()
+{
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    for (int k = 0; k < 4; k++) {
+        if (D.f[k]) (void)hipFree(D.f[k]);
+        D.f[k] = nullptr;
+    }
+    if (D.coef) (void)hipFree(D.coef);
+    if (D.dvol) (void)hipFree(D.dvol);
+    if (D.trace) (void)hipFree(D.trace);
+    if (D.ctr) (void)hipFree(D.ctr);
+    if (D.src_tab) (void)hipFree(D.src_tab);
+    D = Device();
+}
+
+static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, float dz, float dt,
+                      const float *vpz, const float *vsv, const float *eps, const float *del,
+                      const float *phi, const float *theta, const float *pp, const float *pc,
+                      const float *qp, const float *qc)
+{
+    if (bord < 4) return fail(FL_ERR_ARG, "bord=%d: the 8th-order stencil needs bord >= 4", bord);
+    if (sx <= 2 * bord || sy <= 2 * bord || sz <= 2 * bord)
+        return fail(FL_ERR_ARG, "grid %dx%dx%d has no interior for bord=%d", sx, sy, sz, bord);
+    if (!(dx > 0 && dy > 0 && dz > 0 && dt > 0))
+        return fail(FL_ERR_ARG, "spacings and dt must be positive");
+    if (!vpz || !vsv || !eps || !del || !phi || !theta)
+        return fail(FL_ERR_ARG, "model arrays must not be NULL");
+    int rc = ensure_stream();
+    if (rc) return rc;
+    free_device();
+    const long long n = (long long)sx * sy * sz;
+    D.sx = sx; D.sy = sy; D.sz = sz; D.bord = bord;
+    D.dx = dx; D.dy = dy; D.dz = dz; D.dt = dt;
+    D.n = n;
+    D.cm = g_precision;
+    for (int k = 0; k < 4; k++) {
+        if ((rc = dev_alloc((void **)&D.f[k], (size_t)n * 4, &D.bytes))) { free_device(); return rc; }
+    }
+    const float *host[4] = {pp, pc, qp, qc};
+    for (int k = 0; k < 4; k++) {
+        hipError_t e = host[k] ? hipMemcpy(D.f[k], host[k], (size_t)n * 4, hipMemcpyHostToDevice)
+                               : hipMemset(D.f[k], 0, (size_t)n * 4);
+        if (e != hipSuccess) { free_device(); return fail(FL_ERR_HIP, "field upload: %s", hipGetErrorString(e)); }
+    }
+    const size_t cbytes = (size_t)coef_count(D.cm) * n * coef_elem(D.cm);
+    if ((rc = dev_alloc(&D.coef, cbytes, &D.bytes))) { free_device(); return rc; }
+    if ((rc = upload_coefs(vpz, vsv, eps, del, phi, theta, dt, D.cm, D.coef, n, (long long)sx * sy, sz))) {
+        free_device();
+        return rc;
+    }
+    if ((rc = dev_alloc((void **)&D.ctr, 2 * sizeof(int), &D.bytes))) { free_device(); return rc; }
+    D.init = true;
+    return FL_OK;
+}
+
+extern "C" void DRIVER_Initialize(const int sx, const int sy, const int sz, const int bord, float dx,
+                                  float dy, float dz, float dt, float *__restrict__ vpz,
+                                  float *__restrict__ vsv, float *__restrict__ epsilon,
+                                  float *__restrict__ delta, float *__restrict__ phi,
+                                  float *__restrict__ theta, float *__restrict__ pp,
+                                  float *__restrict__ pc, float *__restrict__ qp,
+                                  float *__restrict__ qc)
+{
+    if (initialize(sx, sy, sz, bord, dx, dy, dz, dt, vpz, vsv, epsilon, delta, phi, theta, pp, pc, qp, qc) == FL_OK)
+        fl_clear_error();
+}
+
+extern "C" void DRIVER_Finalize() { free_device(); }
+
+static int propagate(int sx, int sy, int sz, int bord, int it)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "DRIVER_Propagate before DRIVER_Initialize");
+    if (sx != D.sx || sy != D.sy || sz != D.sz || bord != D.bord)
+        return fail(FL_ERR_ARG, "grid %dx%dx%d/%d differs from the initialised %dx%dx%d/%d", sx, sy, sz,
+                    bord, D.sx, D.sy, D.sz, D.bord);
+    StepArgs a = base_args();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g_timing) {
+        int rc = take_events(&e0, &e1);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(e0, g_stream));
+    }
+    int rc = launch_step(a, it, g_stream);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    if (g_timing) {
+        HIPCHK(hipEventRecord(e1, g_stream));
+        g_ev_used.push_back({e0, e1});
+    }
+    swap_levels();
+    if (g_sync) HIPCHK(hipStreamSynchronize(g_stream));
+    return FL_OK;
+}
+
+extern "C" void DRIVER_Propagate(const int sx, const int sy, const int sz, const int bord, const float dx,
+                                 const float dy, const float dz, const float dt, const int it, float *pp,
+                                 float *pc, float *qp, float *qc)
+{
+    (void)dx; (void)dy; (void)dz; (void)dt; (void)pp; (void)pc; (void)qp; (void)qc;
+    propagate(sx, sy, sz, bord, it);
+}
+
+static int insert_src(int iSource, float src)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "DRIVER_InsertSource before DRIVER_Initialize");
+    if (iSource < 0 || (long long)iSource >= D.n)
+        return fail(FL_ERR_ARG, "iSource=%d outside the %lld-cell grid", iSource, D.n);
+    hipLaunchKernelGGL(insert_source, dim3(1), dim3(64), 0, g_stream, D.f[1], D.f[3],
+                       (long long)iSource, src);
+    HIPCHK(hipGetLastError());
+    if (g_sync) HIPCHK(hipStreamSynchronize(g_stream));
+    return FL_OK;
+}
+
+extern "C" void DRIVER_InsertSource(float dt, int it, int iSource, float *p, float *q, float src)
+{
+    (void)dt; (void)it; (void)p; (void)q;
+    insert_src(iSource, src);
+}
+
+static int update_pointers(int sx, int sy, int sz, float *pc)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "DRIVER_Update_pointers before DRIVER_Initialize");
+    if ((long long)sx * sy * sz != D.n) return fail(FL_ERR_ARG, "grid size differs from the initialised one");
+    if (!pc) return FL_OK;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpy(pc, D.f[1], (size_t)D.n * 4, hipMemcpyDeviceToHost));
+    return FL_OK;
+}
+
+extern "C" void DRIVER_Update_pointers(const int sx, const int sy, const int sz, float *pc)
+{
+    update_pointers(sx, sy, sz, pc);
+}
+
+// ------------------------------------------------------------------ extensions
+extern "C" int fl_set_variant(int v)
+{
+    if (v < FL_VARIANT_FUSED || v > FL_VARIANT_TWOPASS) return fail(FL_ERR_ARG, "unknown variant %d", v);
+    g_variant = v;
+    return FL_OK;
+}
+extern "C" int fl_get_variant(void) { return g_variant; }
+
+extern "C" int fl_set_precision(int p)
+{
+    if (p < FL_PREC_FP32 || p > FL_PREC_FP16_RECOMPUTE) return fail(FL_ERR_ARG, "unknown precision %d", p);
+    g_precision = p;
+    return FL_OK;
+}
+extern "C" int fl_get_precision(void) { return D.init ? D.cm : g_precision; }
+
+extern "C" int fl_set_sync(int on)
+{
+    g_sync = on != 0;
+    return FL_OK;
+}
+
+extern "C" int fl_set_tiles(int ty, int zt)
+{
+    if (ty == 0) ty = 8;
+    if (ty != 4 && ty != 8 && ty != 16) return fail(FL_ERR_ARG, "tile rows must be 4, 8 or 16 (got %d)", ty);
+    if (zt < 0) return fail(FL_ERR_ARG, "zt must be >= 0");
+    g_tiles.ty = ty;
+    g_tiles.zt = zt;
+    return FL_OK;
+}
+
+extern "C" int fl_get_tiles(int *ty, int *zt)
+{
+    if (ty) *ty = g_tiles.ty;
+    if (zt) *zt = (g_tiles.zt > 0 || !D.init) ? g_tiles.zt : default_zt(D.sx, D.sy, D.sz, D.bord, g_tiles.ty);
+    return FL_OK;
+}
+
+extern "C" int fl_get_field(int which, float *host)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "no initialised grid");
+    if (which < 0 || which > 3 || !host) return fail(FL_ERR_ARG, "bad field request %d", which);
+    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpy(host, D.f[which], (size_t)D.n * 4, hipMemcpyDeviceToHost));
+    return FL_OK;
+}
+
+extern "C" int fl_set_field(int which, const float *host)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "no initialised grid");
+    if (which < 0 || which > 3 || !host) return fail(FL_ERR_ARG, "bad field request %d", which);
+    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpy(D.f[which], host, (size_t)D.n * 4, hipMemcpyHostToDevice));
+    return FL_OK;
+}
+
+extern "C" int fl_record_traces(int iz, int every, int max_traces)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "no initialised grid");
+    if (D.trace) {
+        HIPCHK(hipStreamSynchronize(g_stream));
+        (void)hipFree(D.trace);
+        D.bytes -= (long long)D.rec_slots * (D.sx - 2 * D.bord) * (D.sy - 2 * D.bord) * 4;
+        D.trace = nullptr;
+    }
+    D.rec_iz = -1; D.rec_every = 0; D.rec_slots = 0;
+    if (iz < 0) return FL_OK;
+    if (iz < D.bord || iz >= D.sz - D.bord || every <= 0 || max_traces <= 0)
+        return fail(FL_ERR_ARG, "receiver plane %d / every %d / slots %d invalid", iz, every, max_traces);
+    const size_t bytes = (size_t)max_traces * (D.sx - 2 * D.bord) * (D.sy - 2 * D.bord) * 4;
+    int rc = dev_alloc((void **)&D.trace, bytes, &D.bytes);
+    if (rc) return rc;
+    HIPCHK(hipMemset(D.trace, 0, bytes));
+    D.rec_iz = iz; D.rec_every = every; D.rec_slots = max_traces;
+    return FL_OK;
+}
+
+extern "C" int fl_get_traces(float *host, int max_traces)
+{
+    if (!D.init || !D.trace) return fail(FL_ERR_STATE, "traces are not being recorded");
+    const int m = std::min(max_traces, D.rec_slots);
+    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpy(host, D.trace, (size_t)m * (D.sx - 2 * D.bord) * (D.sy - 2 * D.bord) * 4,
+                     hipMemcpyDeviceToHost));
+    return m;
+}
+
+__global__ void set_int(int *p, int v) { if (threadIdx.x == 0) *p = v; }
+
+extern "C" int fl_run_steps(int it0, int nsteps, int iSource, const float *src, int graph_chunk)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "fl_run_steps before DRIVER_Initialize");
+    if (nsteps <= 0) return FL_OK;
+    if (!src) return fail(FL_ERR_ARG, "src must not be NULL");
+    if (iSource < 0 || (long long)iSource >= D.n) return fail(FL_ERR_ARG, "iSource outside the grid");
+    // the captured form adds the source when the kernel stores its cell: only cells of
+    // the update domain are stored, so a source elsewhere takes the call-by-call path
+    const long long sxy = (long long)D.sx * D.sy;
+    const int siz = (int)(iSource / sxy), siy = (int)((iSource % sxy) / D.sx), six = (int)(iSource % D.sx);
+    const bool src_in_domain = siz >= D.bord && siz < D.sz - D.bord && siy >= D.bord &&
+                               siy < D.sy - D.bord && six >= D.bord && six < D.sx - D.bord;
+    if (g_variant != FL_VARIANT_FUSED || graph_chunk <= 0 || !src_in_domain) {
+        // plain launches: InsertSource + Propagate per step
+        const bool sync = g_sync;
+        g_sync = false;
+        int rc = FL_OK;
+        for (int k = 0; k < nsteps && rc == FL_OK; k++) {
+            rc = insert_src(iSource, src[k]);
+            if (rc == FL_OK) rc = propagate(D.sx, D.sy, D.sz, D.bord, it0 + 1 + k);
+        }
+        g_sync = sync;
+        if (rc) return rc;
+        if (g_sync) HIPCHK(hipStreamSynchronize(g_stream));
+        return FL_OK;
+    }
+    if (D.src_cap < nsteps) {
+        if (D.src_tab) (void)hipFree(D.src_tab);
+        D.src_tab = nullptr;
+        int rc = dev_alloc((void **)&D.src_tab, (size_t)nsteps * 4, &D.bytes);
+        if (rc) return rc;
+        D.src_cap = nsteps;
+    }
+    HIPCHK(hipMemcpyAsync(D.src_tab, src, (size_t)nsteps * 4, hipMemcpyHostToDevice, g_stream));
+    hipLaunchKernelGGL(set_int, dim3(1), dim3(64), 0, g_stream, D.ctr, 0);
+    hipLaunchKernelGGL(insert_source, dim3(1), dim3(64), 0, g_stream, D.f[1], D.f[3], (long long)iSource, src[0]);
+    const int K = std::min(nsteps, (graph_chunk + 1) / 2 * 2);  // even: levels return to their roles
+    auto step_args = [&](int k) {
+        StepArgs a = base_args();
+        a.it_base = it0;
+        a.it_dev = D.ctr + (k & 1);
+        a.it_next = D.ctr + ((k + 1) & 1);
+        a.src_tab = D.src_tab;
+        a.src_n = nsteps;
+        a.src_idx = iSource;
+        return a;
+    };
+    int done = 0;
+    if (K >= 2 && nsteps >= K) {
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+        int rc = FL_OK;
+        for (int k = 0; k < K && rc == FL_OK; k++) {
+            rc = launch_zmarch(step_args(k), g_tiles.ty, D.cm, false, g_stream);
+            swap_levels();
+        }
+        hipError_t ec = hipStreamEndCapture(g_stream, &graph);
+        if (rc) return rc;
+        if (ec != hipSuccess) return fail(FL_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        const int reps = nsteps / K;
+        for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(exec, g_stream));
+        done = reps * K;
+        HIPCHK(hipStreamSynchronize(g_stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    }
+    for (int k = done; k < nsteps; k++) {
+        int rc = launch_zmarch(step_args(k), g_tiles.ty, D.cm, false, g_stream);
+        if (rc) return rc;
+        swap_levels();
+    }
+    HIPCHK(hipGetLastError());
+    if (g_sync) HIPCHK(hipStreamSynchronize(g_stream));
+    return FL_OK;
+}
+
+extern "C" int fl_synchronize(void)
+{
+    if (!g_stream) return FL_OK;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return FL_OK;
+}
+
+extern "C" int fl_set_timing(int on)
+{
+    g_timing = on != 0;
+    return FL_OK;
+}
+
+extern "C" int fl_kernel_times(double *total_ms, int *count)
+{
+    double tot = 0.0;
+    int cnt = 0;
+    if (!g_ev_used.empty()) {
+        HIPCHK(hipStreamSynchronize(g_stream));
+        for (auto &p : g_ev_used) {
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, p.first, p.second));
+            tot += ms;
+            cnt++;
+            g_ev_free.push_back(p);
+        }
+        g_ev_used.clear();
+    }
+    if (total_ms) *total_ms = tot;
+    if (count) *count = cnt;
+    return FL_OK;
+}
+
+extern "C" int fl_region_begin(void)
+{
+    int rc = ensure_stream();
+    if (rc) return rc;
+    if (!g_reg0) {
+        HIPCHK(hipEventCreate(&g_reg0));
+        HIPCHK(hipEventCreate(&g_reg1));
+    }
+    HIPCHK(hipEventRecord(g_reg0, g_stream));
+    return FL_OK;
+}
+
+extern "C" int fl_region_end(double *ms)
+{
+    if (!g_reg0) return fail(FL_ERR_STATE, "fl_region_end without fl_region_begin");
+    HIPCHK(hipEventRecord(g_reg1, g_stream));
+    HIPCHK(hipEventSynchronize(g_reg1));
+    float m = 0.0f;
+    HIPCHK(hipEventElapsedTime(&m, g_reg0, g_reg1));
+    if (ms) *ms = m;
+    return FL_OK;
+}
+
+extern "C" int fl_device_name(char *buf, int len)
+{
+    int rc = ensure_stream();
+    if (rc) return rc;
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, 0));
+    snprintf(buf, len, "%s %s", p.name, p.gcnArchName);
+    return FL_OK;
+}
+
+extern "C" long long fl_device_bytes(void) { return D.bytes; }
+
+// ------------------------------------------------------------------ autotuner
+static bool cache_lookup(const char *path, const std::string &key, int *ty, int *zt)
+{
+    if (!path) return false;
+    FILE *f = fopen(path, "r");
+    if (!f) return false;
+    char line[512];
+    bool found = false;
+    while (fgets(line, sizeof(line), f)) {
+        char k[400];
+        int a, b;
+        if (sscanf(line, "%399s %d %d", k, &a, &b) == 3 && key == k) {
+            *ty = a; *zt = b;
+            found = true;  // last entry wins
+        }
+    }
+    fclose(f);
+    return found;
+}
+
+static void cache_store(const char *path, const std::string &key, int ty, int zt, double ms)
+{
+    if (!path) return;
+    FILE *f = fopen(path, "a");
+    if (!f) return;
+    fprintf(f, "%s %d %d %.4f\n", key.c_str(), ty, zt, ms);
+    fclose(f);
+}
+
+static std::string tune_key(const char *kind, int sx, int sy, int sz, int cm)
+{
+    char name[256] = "";
+    fl_device_name(name, sizeof(name));
+    for (char *c = name; *c; c++)
+        if (*c == ' ') *c = '_';
+    char buf[400];
+    snprintf(buf, sizeof(buf), "%s|%s|%dx%dx%d|v%d|p%d", name, kind, sx, sy, sz, g_variant, cm);
+    return buf;
+}
+
+static std::vector<int> zt_candidates(int nzu)
+{
+    std::vector<int> c = {16, 24, 32, 48, 64, 96, 128, 192};
+    std::vector<int> r;
+    for (int z : c)
+        if (z < nzu) r.push_back(z);
+    r.push_back(nzu);
+    return r;
+}
+
+extern "C" int fl_autotune(int steps, const char *cache_path)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "fl_autotune before DRIVER_Initialize");
+    if (g_variant == FL_VARIANT_DIRECT) return FL_OK;  // no tile parameters
+    const std::string key = tune_key("grid", D.sx, D.sy, D.sz, D.cm);
+    int ty, zt;
+    if (cache_lookup(cache_path, key, &ty, &zt)) return fl_set_tiles(ty, zt);
+    steps = std::max(1, steps);
+    // keep the fields: tuning steps run on them and are undone afterwards
+    float *save = nullptr;
+    long long dummy = 0;
+    int rc = dev_alloc((void **)&save, (size_t)4 * D.n * 4, &dummy);
+    if (rc) return rc;
+    float *orig[4] = {D.f[0], D.f[1], D.f[2], D.f[3]};
+    for (int k = 0; k < 4; k++)
+        HIPCHK(hipMemcpyAsync(save + k * D.n, D.f[k], (size_t)D.n * 4, hipMemcpyDeviceToDevice, g_stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const Tiles keep = g_tiles;
+    double best = 1e30;
+    Tiles bt = keep;
+    for (int cty : {4, 8, 16}) {
+        for (int czt : zt_candidates(D.sz - 2 * D.bord)) {
+            g_tiles.ty = cty;
+            g_tiles.zt = czt;
+            StepArgs a = base_args();
+            a.trace = nullptr;
+            rc = launch_step(a, 0, g_stream);  // warm-up
+            if (rc) break;
+            HIPCHK(hipEventRecord(e0, g_stream));
+            for (int s = 0; s < steps && rc == FL_OK; s++) {
+                StepArgs b = base_args();
+                b.trace = nullptr;
+                rc = launch_step(b, 0, g_stream);
+                swap_levels();
+            }
+            HIPCHK(hipEventRecord(e1, g_stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (rc) break;
+            if (ms < best) {
+                best = ms;
+                bt = g_tiles;
+            }
+        }
+        if (rc) break;
+    }
+    for (int k = 0; k < 4; k++) {
+        D.f[k] = orig[k];
+        HIPCHK(hipMemcpyAsync(D.f[k], save + k * D.n, (size_t)D.n * 4, hipMemcpyDeviceToDevice, g_stream));
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    (void)hipFree(save);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        g_tiles = keep;
+        return rc;
+    }
+    g_tiles = bt;
+    cache_store(cache_path, key, bt.ty, bt.zt, best / steps);
+    return FL_OK;
+}
+
+#include "fl_batch.inc"

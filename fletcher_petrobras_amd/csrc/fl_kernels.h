@@ -1,0 +1,352 @@
+// Device code of the Fletcher TTI time step for gfx950 (CDNA4, wave64).
+//
+// One time step updates the coupled pseudo-acoustic fields p, q (reference
+// original/sample.h:32-90 and der1der1/sample.h:35-123):
+//   pp <- 2 pc - pp + dt^2 rhsp(pc, qc),   qp <- 2 qc - qp + dt^2 rhsq(pc, qc)
+// over iz,iy,ix in [bord, s-bord) (original/OpenMP/openmp_propagate.c:27-29).
+//
+// Kernels:
+//   zmarch<TY, CM, TWOPASS>  the drop-in: a 64 x TY xy tile per workgroup marches
+//       z over a chunk.  The incoming plane f = z+4 of pc/qc is staged in LDS with
+//       a 4-cell halo; first derivatives Dx, Dy of that plane are formed on chip;
+//       9-deep register queues hold pc, qc, Dx, Dy along z, and a 4-deep queue
+//       delays the in-plane terms (Dxx, Dyy, Dxy) until plane z is complete.
+//       Mixed derivatives are Der1 of Der1 (der1der1/sample.h:57-62, 83-88);
+//       TWOPASS reads Dx, Dy from the volumes written by der1_pass instead.
+//       CM selects the coefficient storage (FP32 / BF16 / FP16 / FP16 recompute).
+//   direct_dercross  the reference's direct 64-point DerCross form
+//       (original/derivatives.h:59-68), one thread per (x,y) column.
+//   der1_pass        pass 1 of the two-pass form (der1der1/OpenMP/openmp_propagate.c:24-76).
+//   insert_source    p[i] += s, q[i] += s (original/OpenMP/openmp_insertsource.c:7-12).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fl {
+
+// 8th-order weights: the reference's float literals (original/derivatives.h:13-41).
+#define FL_L1 0.8f
+#define FL_L2 -0.2f
+#define FL_L3 0.0380952380952381f
+#define FL_L4 -0.0035714285714285713f
+#define FL_L11 0.64f
+#define FL_L12 -0.16f
+#define FL_L13 0.03047619047619047618f
+#define FL_L14 -0.00285714285714285713f
+#define FL_L22 0.04f
+#define FL_L23 -0.00761904761904761904f
+#define FL_L24 0.00071428571428571428f
+#define FL_L33 0.00145124716553287981f
+#define FL_L34 -0.00013605442176870748f
+#define FL_L44 0.00001275510204081632f
+#define FL_K0 -2.84722222222222222222f
+#define FL_K1 1.6f
+#define FL_K2 -0.2f
+#define FL_K3 0.02539682539682539682f
+#define FL_K4 -0.00178571428571428571f
+
+// Coefficient storage modes.
+enum CoefMode : int {
+    CM_F32 = 0,   // 10 FP32 volumes, precomp.h layout
+    CM_BF16 = 1,  // 10 BF16 volumes; v2* pre-multiplied by dt^2
+    CM_F16 = 2,   // 10 FP16 volumes; v2* pre-multiplied by dt^2
+    CM_F16R = 3,  // 6 FP16 volumes (vpz*dt, vsv*dt, eps, delta, theta, phi-pi), recomputed per cell
+};
+
+struct StepArgs {
+    float *pp;
+    const float *pc;
+    float *qp;
+    const float *qc;
+    const void *coef;     // coefficient volumes, consecutive, coef_stride elements apart
+    long long coef_stride;
+    const float *dvol;    // TWOPASS: pDx, pDy, qDx, qDy volumes, n elements apart
+    float dxxinv, dyyinv, dzzinv, dxinv, dyinv, dzinv, dt;
+    int sx, sy, sz, bord;
+    int ntx, nty, zt, nblk;  // tiles along x, y; z chunk depth; logical blocks
+    // step index it: it_host, or it_base + 1 + *it_dev inside a captured run, where
+    // the kernel also publishes *it_next = *it_dev + 1 for the following step
+    int it_host, it_base;
+    const int *it_dev;
+    int *it_next;
+    // source of the next step added to this step's output (captured runs)
+    const float *src_tab;
+    int src_n;
+    long long src_idx;
+    // receiver plane: slot it/rec_every-1 of trace when it % rec_every == 0
+    float *trace;
+    int rec_iz, rec_every, rec_slots;
+};
+
+// Per-launch scalars resolved from StepArgs at kernel entry.
+struct StepCtl {
+    float *trace;   // plane to write this step, or nullptr
+    float src_add;
+    bool do_src;
+};
+
+__device__ __forceinline__ StepCtl step_ctl(const StepArgs &a)
+{
+    StepCtl c;
+    int k = 0, it = a.it_host;
+    if (a.it_dev) {
+        k = *a.it_dev;
+        it = a.it_base + 1 + k;
+    }
+    c.do_src = a.src_tab && (k + 1) < a.src_n;
+    c.src_add = c.do_src ? a.src_tab[k + 1] : 0.0f;
+    c.trace = nullptr;
+    if (a.trace && a.rec_every > 0 && it % a.rec_every == 0) {
+        const int slot = it / a.rec_every - 1;
+        if (slot >= 0 && slot < a.rec_slots)
+            c.trace = a.trace + (long long)slot * (a.sx - 2 * a.bord) * (a.sy - 2 * a.bord);
+    }
+    return c;
+}
+
+__device__ __forceinline__ float der1_lds(const float *p, int s, float dinv)
+{
+    return (FL_L1 * (p[s] - p[-s]) + FL_L2 * (p[2 * s] - p[-2 * s]) +
+            FL_L3 * (p[3 * s] - p[-3 * s]) + FL_L4 * (p[4 * s] - p[-4 * s])) *
+           dinv;
+}
+
+__device__ __forceinline__ float der2_lds(const float *p, int s, float d2inv)
+{
+    return (FL_K0 * p[0] + FL_K1 * (p[s] + p[-s]) + FL_K2 * (p[2 * s] + p[-2 * s]) +
+            FL_K3 * (p[3 * s] + p[-3 * s]) + FL_K4 * (p[4 * s] + p[-4 * s])) *
+           d2inv;
+}
+
+// q[0..8] = values at z-4..z+4
+__device__ __forceinline__ float der1_q(const float *q, float dinv)
+{
+    return (FL_L1 * (q[5] - q[3]) + FL_L2 * (q[6] - q[2]) + FL_L3 * (q[7] - q[1]) +
+            FL_L4 * (q[8] - q[0])) *
+           dinv;
+}
+
+__device__ __forceinline__ float der2_q(const float *q, float d2inv)
+{
+    return (FL_K0 * q[4] + FL_K1 * (q[5] + q[3]) + FL_K2 * (q[6] + q[2]) + FL_K3 * (q[7] + q[1]) +
+            FL_K4 * (q[8] + q[0])) *
+           d2inv;
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ float f16_to_f32(uint16_t h)
+{
+    _Float16 v = __builtin_bit_cast(_Float16, h);
+    return (float)v;
+}
+
+// The 10 per-cell coefficients, in precomp.h order; v2* include dt^2 when `scaled`.
+struct Coef {
+    float c[10];
+};
+
+template <int CM>
+__device__ __forceinline__ void load_coef(const void *base, long long stride, long long i, Coef &k)
+{
+    if constexpr (CM == CM_F32) {
+        const float *b = (const float *)base;
+#pragma unroll
+        for (int j = 0; j < 10; j++) k.c[j] = __builtin_nontemporal_load(b + j * stride + i);
+    } else if constexpr (CM == CM_BF16 || CM == CM_F16) {
+        const uint16_t *b = (const uint16_t *)base;
+#pragma unroll
+        for (int j = 0; j < 10; j++) {
+            const uint16_t h = __builtin_nontemporal_load(b + j * stride + i);
+            k.c[j] = (CM == CM_BF16) ? bf16_to_f32(h) : f16_to_f32(h);
+        }
+    } else {  // CM_F16R: recompute from vpz*dt, vsv*dt, eps, delta, theta, phi-pi
+        const uint16_t *b = (const uint16_t *)base;
+        float r[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) r[j] = f16_to_f32(__builtin_nontemporal_load(b + j * stride + i));
+        const float w2 = r[0] * r[0];
+        float st, ct, spc, cpc;
+        __sincosf(r[4], &st, &ct);
+        __sincosf(r[5], &spc, &cpc);
+        const float sp = -spc, cp = -cpc;  // phi = phic + pi
+        const float st2 = st * st;
+        const float s2t = 2.0f * st * ct;
+        const float s2p = 2.0f * sp * cp;
+        k.c[0] = st2 * cp * cp;
+        k.c[1] = st2 * sp * sp;
+        k.c[2] = ct * ct;
+        k.c[3] = st2 * s2p;
+        k.c[4] = s2t * sp;
+        k.c[5] = s2t * cp;
+        k.c[6] = w2 * (1.0f + 2.0f * r[2]);
+        k.c[7] = w2;
+        k.c[8] = r[1] * r[1];
+        k.c[9] = w2 * (1.0f + 2.0f * r[3]);
+    }
+}
+
+// Coefficient combination + leapfrog, association order of original/sample.h:46-86.
+template <int CM>
+__device__ __forceinline__ void combine(const Coef &k, float dt, float pxx, float pyy, float pzz,
+                                        float pxy, float pyz, float pxz, float qxx, float qyy,
+                                        float qzz, float qxy, float qyz, float qxz, float pc,
+                                        float qc, float &pp, float &qp)
+{
+    const float h1p = k.c[0] * pxx + k.c[1] * pyy + k.c[2] * pzz + k.c[3] * pxy + k.c[5] * pxz +
+                      k.c[4] * pyz;
+    const float h2p = pxx + pyy + pzz - h1p;
+    const float h1q = k.c[0] * qxx + k.c[1] * qyy + k.c[2] * qzz + k.c[3] * qxy + k.c[5] * qxz +
+                      k.c[4] * qyz;
+    const float h2q = qxx + qyy + qzz - h1q;
+    const float h1pmq = h1p - h1q;
+    const float h2pmq = h2p - h2q;
+    const float rhsp = k.c[6] * h2p + k.c[7] * h1q + k.c[8] * h1pmq;
+    const float rhsq = k.c[9] * h2p + k.c[7] * h1q - k.c[8] * h2pmq;
+    if constexpr (CM == CM_F32) {
+        pp = 2.0f * pc - pp + rhsp * dt * dt;
+        qp = 2.0f * qc - qp + rhsq * dt * dt;
+    } else {  // dt^2 folded into the stored v2* factors
+        pp = 2.0f * pc - pp + rhsp;
+        qp = 2.0f * qc - qp + rhsq;
+    }
+}
+
+// Logical block id from the hardware one.  Workgroups are dealt to the 8 XCDs
+// round-robin (block b runs on XCD b%8); this gives each XCD a contiguous range of
+// logical tiles so neighbouring tiles (shared halos) meet in the same L2.
+__device__ __forceinline__ int xcd_remap(int b, int g)
+{
+    return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------------
+// Direct 64-point DerCross form (original/sample.h:32-90), global loads through L1/L2.
+__device__ __forceinline__ float cross_group(const float *__restrict__ p, long long i, long long s11,
+                                             long long s21, int m, int n)
+{
+    const long long aa = n * s11, bb = m * s21;
+    float v = p[i + bb + aa] - p[i + bb - aa] - p[i - bb + aa] + p[i - bb - aa];
+    if (m != n) {
+        const long long a2 = m * s11, b2 = n * s21;
+        v = v + p[i + b2 + a2] - p[i + b2 - a2] - p[i - b2 + a2] + p[i - b2 - a2];
+    }
+    return v;
+}
+
+__device__ __forceinline__ float dercross_g(const float *__restrict__ p, long long i, long long s11,
+                                            long long s21, float dinv)
+{
+    return (FL_L11 * cross_group(p, i, s11, s21, 1, 1) + FL_L12 * cross_group(p, i, s11, s21, 1, 2) +
+            FL_L13 * cross_group(p, i, s11, s21, 1, 3) + FL_L14 * cross_group(p, i, s11, s21, 1, 4) +
+            FL_L22 * cross_group(p, i, s11, s21, 2, 2) + FL_L23 * cross_group(p, i, s11, s21, 2, 3) +
+            FL_L24 * cross_group(p, i, s11, s21, 2, 4) + FL_L33 * cross_group(p, i, s11, s21, 3, 3) +
+            FL_L34 * cross_group(p, i, s11, s21, 3, 4) + FL_L44 * cross_group(p, i, s11, s21, 4, 4)) *
+           dinv;
+}
+
+__device__ __forceinline__ float der2_g(const float *__restrict__ p, long long i, long long s, float d2inv)
+{
+    return (FL_K0 * p[i] + FL_K1 * (p[i + s] + p[i - s]) + FL_K2 * (p[i + 2 * s] + p[i - 2 * s]) +
+            FL_K3 * (p[i + 3 * s] + p[i - 3 * s]) + FL_K4 * (p[i + 4 * s] + p[i - 4 * s])) *
+           d2inv;
+}
+
+__device__ __forceinline__ float der1_g(const float *__restrict__ p, long long i, long long s, float dinv)
+{
+    return (FL_L1 * (p[i + s] - p[i - s]) + FL_L2 * (p[i + 2 * s] - p[i - 2 * s]) +
+            FL_L3 * (p[i + 3 * s] - p[i - 3 * s]) + FL_L4 * (p[i + 4 * s] - p[i - 4 * s])) *
+           dinv;
+}
+
+struct DirectArgs {
+    float *pp;
+    const float *pc;
+    float *qp;
+    const float *qc;
+    const float *coef;
+    long long coef_stride;
+    float dxxinv, dyyinv, dzzinv, dxyinv, dxzinv, dyzinv, dt;
+    int sx, sy, sz, bord, zt;
+    float *trace;
+    int rec_iz;
+};
+
+// grid: (ceil(nx/64), ceil(ny/4), nzchunks); block (64, 4)
+__global__ __launch_bounds__(256) void direct_dercross(const DirectArgs a)
+{
+    const int x = a.bord + blockIdx.x * 64 + threadIdx.x;
+    const int y = a.bord + blockIdx.y * 4 + threadIdx.y;
+    if (x >= a.sx - a.bord || y >= a.sy - a.bord) return;
+    const int zlo = a.bord + blockIdx.z * a.zt;
+    const int zhi = min(zlo + a.zt, a.sz - a.bord);
+    const long long sX = 1, sY = a.sx, sZ = (long long)a.sx * a.sy;
+    for (int z = zlo; z < zhi; z++) {
+        const long long i = z * sZ + (long long)y * a.sx + x;
+        const float pxx = der2_g(a.pc, i, sX, a.dxxinv);
+        const float pyy = der2_g(a.pc, i, sY, a.dyyinv);
+        const float pzz = der2_g(a.pc, i, sZ, a.dzzinv);
+        const float pxy = dercross_g(a.pc, i, sX, sY, a.dxyinv);
+        const float pyz = dercross_g(a.pc, i, sY, sZ, a.dyzinv);
+        const float pxz = dercross_g(a.pc, i, sX, sZ, a.dxzinv);
+        const float qxx = der2_g(a.qc, i, sX, a.dxxinv);
+        const float qyy = der2_g(a.qc, i, sY, a.dyyinv);
+        const float qzz = der2_g(a.qc, i, sZ, a.dzzinv);
+        const float qxy = dercross_g(a.qc, i, sX, sY, a.dxyinv);
+        const float qyz = dercross_g(a.qc, i, sY, sZ, a.dyzinv);
+        const float qxz = dercross_g(a.qc, i, sX, sZ, a.dxzinv);
+        Coef k;
+#pragma unroll
+        for (int j = 0; j < 10; j++) k.c[j] = a.coef[j * a.coef_stride + i];
+        float ppv = a.pp[i], qpv = a.qp[i];
+        combine<CM_F32>(k, a.dt, pxx, pyy, pzz, pxy, pyz, pxz, qxx, qyy, qzz, qxy, qyz, qxz,
+                        a.pc[i], a.qc[i], ppv, qpv);
+        if (a.trace && z == a.rec_iz)
+            a.trace[(long long)(y - a.bord) * (a.sx - 2 * a.bord) + (x - a.bord)] = ppv;
+        a.pp[i] = ppv;
+        a.qp[i] = qpv;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Pass 1 of the two-pass form: Dx of pc,qc over ix in [bord, sx-bord), all iy;
+// Dy over iy in [bord, sy-bord), ix in [bord, sx); every plane iz in [0, sz).
+// grid: (ceil(sx/64), ceil(sy/4), sz); block (64, 4)
+__global__ __launch_bounds__(256) void der1_pass(const float *__restrict__ pc,
+                                                 const float *__restrict__ qc, float *dvol, int sx,
+                                                 int sy, int sz, int bord, float dxinv, float dyinv)
+{
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= sx || y >= sy) return;
+    const long long n = (long long)sx * sy * sz;
+    const long long i = ((long long)z * sy + y) * sx + x;
+    if (x >= bord && x < sx - bord) {
+        dvol[i] = der1_g(pc, i, 1, dxinv);
+        dvol[2 * n + i] = der1_g(qc, i, 1, dxinv);
+    }
+    if (y >= bord && y < sy - bord && x >= bord) {
+        dvol[n + i] = der1_g(pc, i, sx, dyinv);
+        dvol[3 * n + i] = der1_g(qc, i, sx, dyinv);
+    }
+}
+
+__global__ void insert_source(float *p, float *q, long long i, float s)
+{
+    if (threadIdx.x == 0) {
+        p[i] += s;
+        q[i] += s;
+    }
+}
+
+// Receiver-plane gather: pc on plane iz over [bord, sx-bord) x [bord, sy-bord).
+__global__ void gather_plane(const float *__restrict__ pc, float *out, int sx, int sy, int bord, int iz)
+{
+    const int nxr = sx - 2 * bord, nyr = sy - 2 * bord;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nxr * nyr) return;
+    const int iy = e / nxr, ix = e - iy * nxr;
+    out[e] = pc[((long long)iz * sy + iy + bord) * sx + ix + bord];
+}
+
+}  // namespace fl

@@ -1,0 +1,233 @@
+/*
+ * Host model setup for the Fletcher TTI path: the once-per-run work of the
+ * reference front-end (model arrays, random boundary, CFL, Ricker source,
+ * coefficient precompute), restated with the reference's float/double
+ * promotion order so the FP32 results are bitwise those of the reference.
+ * Reference citations are per function.
+ */
+#include "fletcher/fl_model.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define FL_SIGMA 0.75 /* fletcher.h:10 */
+
+/* ------------------------------------------------------------------ rand()
+ * glibc random(): state r[0]=seed, r[i]=16807*r[i-1] mod (2^31-1) for i<31,
+ * r[i]=r[i-31] for 31<=i<34, then r[i]=r[i-31]+r[i-3] (mod 2^32); the k-th
+ * output is r[k+344]>>1.  Kept as a 34-entry ring of the most recent values. */
+void fl_rand_seed(fl_rand_t *s, uint32_t seed)
+{
+    int32_t r[34];
+    r[0] = (int32_t)(seed == 0 ? 1u : seed);
+    for (int i = 1; i < 31; i++) {
+        const int64_t hi = r[i - 1] / 127773, lo = r[i - 1] % 127773;
+        int64_t w = 16807 * lo - 2836 * hi;
+        if (w < 0) w += 2147483647;
+        r[i] = (int32_t)w;
+    }
+    for (int i = 31; i < 34; i++) r[i] = r[i - 31];
+    memcpy(s->r, r, sizeof(r));
+    s->k = 34; /* index of the next value of the recurrence */
+    for (int i = 34; i < 344; i++) (void)fl_rand_next(s);
+}
+
+int32_t fl_rand_next(fl_rand_t *s)
+{
+    const int k = s->k;
+    const uint32_t v = (uint32_t)s->r[(k - 31) % 34] + (uint32_t)s->r[(k - 3) % 34];
+    s->r[k % 34] = (int32_t)v;
+    s->k = (k + 1 >= 68) ? k + 1 - 34 : k + 1; /* k stays in [34,68), k mod 34 preserved */
+    return (int32_t)(v >> 1);
+}
+
+/* ------------------------------------------------------------------ models */
+
+int fl_model_homogeneous(int form, int64_t n, float *vpz, float *vsv, float *epsilon,
+                         float *delta, float *phi, float *theta)
+{
+    /* main.c:168-221 */
+    for (int64_t i = 0; i < n; i++) {
+        switch (form) {
+        case FL_FORM_ISO:
+            vpz[i] = 3000.0;
+            epsilon[i] = 0.0;
+            delta[i] = 0.0;
+            phi[i] = 0.0;
+            theta[i] = 0.0;
+            vsv[i] = 0.0;
+            break;
+        case FL_FORM_VTI:
+            vpz[i] = 3000.0;
+            epsilon[i] = 0.24;
+            delta[i] = 0.1;
+            phi[i] = 0.0;
+            theta[i] = 0.0;
+            vsv[i] = vpz[i] * sqrtf(fabsf(epsilon[i] - delta[i]) / FL_SIGMA);
+            break;
+        case FL_FORM_TTI:
+            vpz[i] = 3000.0;
+            epsilon[i] = 0.24;
+            delta[i] = 0.1;
+            phi[i] = 1.0; /* the reference avoids null coefficients with phi=1 */
+            theta[i] = atanf(1.0);
+            vsv[i] = vpz[i] * sqrtf(fabsf(epsilon[i] - delta[i]) / FL_SIGMA);
+            break;
+        default:
+            return -1;
+        }
+    }
+    return 0;
+}
+
+float fl_recommended_dt(int64_t n, const float *vpz, const float *epsilon, float dx, float dy,
+                        float dz, double mi)
+{
+    /* main.c:225-236: maxvel over vpz*sqrt(1+2eps), recdt = MI*min(d)/maxvel */
+    float maxvel = vpz[0] * sqrt(1.0 + 2 * epsilon[0]);
+    for (int64_t i = 1; i < n; i++) maxvel = fmaxf(maxvel, vpz[i] * sqrt(1.0 + 2 * epsilon[i]));
+    float mindelta = dx;
+    if (dy < mindelta) mindelta = dy;
+    if (dz < mindelta) mindelta = dz;
+    float recdt = (mi * mindelta) / maxvel;
+    return recdt;
+}
+
+int fl_num_steps(float tmax, float dt)
+{
+    return (int)ceil(tmax / dt); /* main.c:101, float quotient */
+}
+
+void fl_random_boundary(uint32_t seed, int sx, int sy, int sz, int nx, int ny, int nz, int bord,
+                        int absorb, float *vpz, float *vsv)
+{
+    /* boundary.c:7-98 */
+#define IND(ix, iy, iz) ((((int64_t)(iz)) * sy + (iy)) * (int64_t)sx + (ix))
+    fl_rand_t rs;
+    fl_rand_seed(&rs, seed);
+    float maxP = 0.0, maxS = 0.0;
+    for (int iz = bord + absorb; iz < nz + bord + absorb; iz++)
+        for (int iy = bord + absorb; iy < ny + bord + absorb; iy++)
+            for (int64_t i = IND(bord + absorb, iy, iz); i < IND(nx + bord + absorb, iy, iz); i++) {
+                maxP = fmaxf(vpz[i], maxP);
+                maxS = fmaxf(vsv[i], maxS);
+            }
+    const int bordLen = bord + absorb - 1; /* last index of the low absorbing zone */
+    const int firstIn = bordLen + 1;       /* first interior index */
+    const float frac = 1.0 / (float)(absorb);
+    for (int iz = 0; iz < sz; iz++)
+        for (int iy = 0; iy < sy; iy++)
+            for (int ix = 0; ix < sx; ix++) {
+                const int64_t i = IND(ix, iy, iz);
+                if ((iz >= firstIn && iz <= bordLen + nz) && (iy >= firstIn && iy <= bordLen + ny) &&
+                    (ix >= firstIn && ix <= bordLen + nx))
+                    continue; /* interior untouched */
+                if ((iz >= bord && iz <= 2 * bordLen + nz) && (iy >= bord && iy <= 2 * bordLen + ny) &&
+                    (ix >= bord && ix <= 2 * bordLen + nx)) {
+                    int distz, disty, distx, ivelz, ively, ivelx;
+                    if (iz > bordLen + nz) { distz = iz - bordLen - nz; ivelz = bordLen + nz; }
+                    else if (iz < firstIn) { distz = firstIn - iz; ivelz = firstIn; }
+                    else { distz = 0; ivelz = iz; }
+                    if (iy > bordLen + ny) { disty = iy - bordLen - ny; ively = bordLen + ny; }
+                    else if (iy < firstIn) { disty = firstIn - iy; ively = firstIn; }
+                    else { disty = 0; ively = iy; }
+                    if (ix > bordLen + nx) { distx = ix - bordLen - nx; ivelx = bordLen + nx; }
+                    else if (ix < firstIn) { distx = firstIn - ix; ivelx = firstIn; }
+                    else { distx = 0; ivelx = ix; }
+                    int dist = (disty > distz) ? disty : distz;
+                    dist = (dist > distx) ? dist : distx;
+                    const float bordDist = (float)(dist)*frac;
+                    const float rfac = (float)fl_rand_next(&rs) / (float)2147483647;
+                    const int64_t j = IND(ivelx, ively, ivelz);
+                    vpz[i] = vpz[j] * (1.0 - bordDist) + maxP * rfac * bordDist;
+                    vsv[i] = vsv[j] * (1.0 - bordDist) + maxS * rfac * bordDist;
+                } else {
+                    vpz[i] = 0.0;
+                    vsv[i] = 0.0;
+                }
+            }
+#undef IND
+}
+
+float fl_source(float dt, int it)
+{
+    /* source.c:7-15 with source.h:11-14; the double constants promote exactly as there */
+    const double FCUT = 40.0, PICUBE = 31.00627668029982017537;
+    const double TWOSQRTPI = 3.54490770181103205458, THREESQRTPI = 5.31736155271654808184;
+    float tf, fc, fct, expo;
+    tf = TWOSQRTPI / FCUT;
+    fc = FCUT / THREESQRTPI;
+    fct = fc * (((float)it) * dt - tf);
+    expo = PICUBE * fct * fct;
+    return ((1.0f - 2.0f * expo) * expf(-expo));
+}
+
+void fl_precompute_f32(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                       const float *delta, const float *phi, const float *theta, float *const *c)
+{
+    /* precomp.h:25-38 and :53-58 */
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const float sinTheta = sin(theta[i]);
+        const float cosTheta = cos(theta[i]);
+        const float sin2Theta = sin(2.0 * theta[i]);
+        const float sinPhi = sin(phi[i]);
+        const float cosPhi = cos(phi[i]);
+        const float sin2Phi = sin(2.0 * phi[i]);
+        c[0][i] = sinTheta * sinTheta * cosPhi * cosPhi;
+        c[1][i] = sinTheta * sinTheta * sinPhi * sinPhi;
+        c[2][i] = cosTheta * cosTheta;
+        c[3][i] = sinTheta * sinTheta * sin2Phi;
+        c[4][i] = sin2Theta * sinPhi;
+        c[5][i] = sin2Theta * cosPhi;
+        const float v2pz = vpz[i] * vpz[i];
+        c[8][i] = vsv[i] * vsv[i];
+        c[7][i] = v2pz;
+        c[6][i] = v2pz * (1.0 + 2.0 * epsilon[i]);
+        c[9][i] = v2pz * (1.0 + 2.0 * delta[i]);
+    }
+}
+
+void fl_precompute_f64(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                       const float *delta, const float *phi, const float *theta, double *const *c)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const double st = sin((double)theta[i]), ct = cos((double)theta[i]);
+        const double s2t = sin(2.0 * theta[i]);
+        const double sp = sin((double)phi[i]), cp = cos((double)phi[i]);
+        const double s2p = sin(2.0 * phi[i]);
+        c[0][i] = st * st * cp * cp;
+        c[1][i] = st * st * sp * sp;
+        c[2][i] = ct * ct;
+        c[3][i] = st * st * s2p;
+        c[4][i] = s2t * sp;
+        c[5][i] = s2t * cp;
+        const double v2pz = (double)vpz[i] * vpz[i];
+        c[8][i] = (double)vsv[i] * vsv[i];
+        c[7][i] = v2pz;
+        c[6][i] = v2pz * (1.0 + 2.0 * epsilon[i]);
+        c[9][i] = v2pz * (1.0 + 2.0 * delta[i]);
+    }
+}
+
+float fl_cfl_dt(int64_t n, const float *vpz, const float *epsilon, float h, float courant)
+{
+    double vmax = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        const double v = vpz[i] * sqrt(1.0 + 2.0 * epsilon[i]);
+        if (v > vmax) vmax = v;
+    }
+    return vmax > 0.0 ? (float)(courant * h / vmax) : 0.0f;
+}
+
+void fl_extract_window(const float *src, int sx, int sy, int sz, int x0, int y0, int z0, int wx,
+                       int wy, int wz, float *dst)
+{
+    (void)sz;
+    for (int iz = 0; iz < wz; iz++)
+        for (int iy = 0; iy < wy; iy++)
+            memcpy(dst + ((int64_t)iz * wy + iy) * wx,
+                   src + (((int64_t)(z0 + iz)) * sy + (y0 + iy)) * sx + x0, sizeof(float) * wx);
+}

@@ -1,0 +1,112 @@
+/*
+ * Extensions of the HIP back-end beyond the reference driver.h: variant and
+ * precision selection, tile shapes and their autotuner, receiver traces,
+ * multi-step runs captured in HIP graphs, device timing, and the batched
+ * multi-shot engine.  C ABI, plain pointers and sizes.
+ *
+ * Every int-returning call returns FL_OK (0) or a negative FL_ERR_* code and
+ * records a message for fl_last_error().
+ */
+#ifndef FLETCHER_FL_HIP_H
+#define FLETCHER_FL_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FL_OK 0
+#define FL_ERR_ARG -1     /* invalid argument (shape, index, mode) */
+#define FL_ERR_STATE -2   /* call out of order (e.g. Propagate before Initialize) */
+#define FL_ERR_HIP -3     /* HIP runtime error */
+#define FL_ERR_NOMEM -4   /* device or host allocation failed */
+
+/* Update schemes for the cross derivatives:
+ *  FUSED   one z-marching kernel, mixed derivatives as Der1 of Der1 formed on chip
+ *          (the arithmetic of der1der1/sample.h:57-62 in a single pass);
+ *  DIRECT  the 64-point DerCross of original/derivatives.h:59-68;
+ *  TWOPASS the reference der1der1 structure: first derivatives written to four
+ *          volumes, then the update (der1der1/OpenMP/openmp_propagate.c:24-93). */
+enum { FL_VARIANT_FUSED = 0, FL_VARIANT_DIRECT = 1, FL_VARIANT_TWOPASS = 2 };
+
+/* Coefficient storage.  FP32 keeps the precomp.h volumes bit for bit; BF16/FP16
+ * store the same 10 volumes (v2* pre-multiplied by dt^2, evaluated in FP64 and
+ * rounded once); FP16_RECOMPUTE stores vpz*dt, vsv*dt, eps, delta, theta, phi-pi
+ * in FP16 and rebuilds the 10 coefficients per cell.  Wavefields stay FP32. */
+enum { FL_PREC_FP32 = 0, FL_PREC_BF16 = 1, FL_PREC_FP16 = 2, FL_PREC_FP16_RECOMPUTE = 3 };
+
+const char *fl_last_error(void);
+int fl_status(void);      /* code of the last failed call, FL_OK after fl_clear_error */
+void fl_clear_error(void);
+
+int fl_set_variant(int variant);   /* takes effect at the next step */
+int fl_get_variant(void);
+int fl_set_precision(int precision); /* used by the next DRIVER_Initialize */
+int fl_get_precision(void);
+int fl_set_sync(int on);           /* 1 (default): DRIVER_Propagate waits for the step */
+
+/* Tile shape of the z-marching kernel: ty rows (4, 8 or 16) of 64 cells, z chunks of
+ * zt planes.  0 selects the built-in default for the grid. */
+int fl_set_tiles(int ty, int zt);
+int fl_get_tiles(int *ty, int *zt);
+
+/* Time every candidate (ty, zt) for `steps` steps on the initialised grid and keep the
+ * fastest; fields are restored afterwards.  When cache_path is not NULL the choice is
+ * looked up / appended there, keyed by device name, grid, variant and precision. */
+int fl_autotune(int steps, const char *cache_path);
+
+/* Device fields after the last step: which = 0 pp, 1 pc, 2 qp, 3 qc (n floats). */
+int fl_get_field(int which, float *host);
+int fl_set_field(int which, const float *host);
+
+/* Receiver traces: after every step it with it % every == 0, pc on plane iz over
+ * [bord, sx-bord) x [bord, sy-bord) is stored in slot it/every - 1 (< max_traces).
+ * iz < 0 disables.  fl_get_traces copies up to max slots and returns the count. */
+int fl_record_traces(int iz, int every, int max_traces);
+int fl_get_traces(float *host, int max_traces);
+
+/* nsteps steps it0+1 .. it0+nsteps of the reference loop (model.c:71-88): before step
+ * it0+1+k, src[k] is added to pc and qc at iSource.  Steps are captured in HIP graphs
+ * of graph_chunk steps (0: no graph) and replayed. */
+int fl_run_steps(int it0, int nsteps, int iSource, const float *src, int graph_chunk);
+
+int fl_synchronize(void);
+
+/* Device timing of the propagate kernels: while on, every step kernel launched by
+ * DRIVER_Propagate is bracketed by HIP events on the library stream.
+ * fl_kernel_times returns the summed milliseconds and the count, then clears. */
+int fl_set_timing(int on);
+int fl_kernel_times(double *total_ms, int *count);
+/* HIP events around a region on the library stream. */
+int fl_region_begin(void);
+int fl_region_end(double *ms);
+
+int fl_device_name(char *buf, int len);
+long long fl_device_bytes(void);   /* device memory held by the back-end */
+/* Algorithmic HBM bytes per updated cell per step of the current configuration. */
+int fl_bytes_per_cell(void);
+
+/* ---------------- batched shots: one persistent launch over (shot, tile) items -------- */
+typedef struct {
+    int sx, sy, sz;   /* array extents of the shot's sub-grid, halo and absorbing zone included */
+    int isource;      /* source cell index within the sub-grid */
+    int rec_iz;       /* receiver plane (-1: none) */
+    const float *vpz, *vsv, *epsilon, *delta, *phi, *theta; /* host model of the sub-grid */
+} fl_shot_t;
+
+int fl_batch_create(int nshots, const fl_shot_t *shots, int bord, float dx, float dy, float dz,
+                    float dt, int precision);
+int fl_batch_set_tiles(int ty, int zt);
+/* Steps it0+1..it0+nsteps for every shot; src as in fl_run_steps; traces every
+ * rec_every steps (0: none) into max_traces slots per shot. */
+int fl_batch_run(int it0, int nsteps, const float *src, int graph_chunk, int rec_every,
+                 int max_traces);
+int fl_batch_get_field(int shot, int which, float *host);
+int fl_batch_get_traces(int shot, float *host, int max_traces);
+int fl_batch_autotune(int steps, const char *cache_path);
+int fl_batch_kernel_ms(double *ms); /* device time of the last fl_batch_run */
+void fl_batch_destroy(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,114 @@
+/*
+ * Host-side model setup of the Fletcher TTI path: the parts of the reference
+ * front-end that run once before the time loop.  C ABI, plain pointers.
+ *
+ * Arrays are FP32 volumes of sx*sy*sz cells, x fastest:
+ *   ind(ix,iy,iz) = ((iz*sy + iy)*sx + ix)        (reference map.h:14)
+ * Each declaration cites the reference code it restates.
+ */
+#ifndef FLETCHER_FL_MODEL_H
+#define FLETCHER_FL_MODEL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Problem formulations of main.c:13 */
+enum { FL_FORM_ISO = 0, FL_FORM_VTI = 1, FL_FORM_TTI = 2 };
+
+/* glibc-compatible rand() stream (TYPE_3 additive feedback generator, the
+ * algorithm behind the reference's unseeded rand() in boundary.c:82), kept
+ * re-entrant so that every shot can own its own stream. */
+typedef struct {
+    int32_t r[34];
+    int k;
+} fl_rand_t;
+void fl_rand_seed(fl_rand_t *s, uint32_t seed);  /* srand(seed) */
+int32_t fl_rand_next(fl_rand_t *s);              /* rand(), in [0, 2^31-1] */
+
+/* Homogeneous ISO / VTI / TTI model — main.c:168-221 (SIGMA=0.75, fletcher.h:10).
+ * Returns 0, or -1 for an unknown form. */
+int fl_model_homogeneous(int form, int64_t n, float *vpz, float *vsv, float *epsilon,
+                         float *delta, float *phi, float *theta);
+
+/* Recommended time step MI*min(dx,dy,dz)/max(vpz*sqrt(1+2*eps)) — main.c:225-236 (MI=0.2). */
+float fl_recommended_dt(int64_t n, const float *vpz, const float *epsilon, float dx, float dy,
+                        float dz, double mi);
+
+/* Random-velocity absorbing boundary — boundary.c:7-98, with the rand() stream
+ * started by srand(seed) (the reference never seeds: seed 1). */
+void fl_random_boundary(uint32_t seed, int sx, int sy, int sz, int nx, int ny, int nz, int bord,
+                        int absorb, float *vpz, float *vsv);
+
+/* Ricker source sample at time it*dt — source.c:7-15, source.h:11-14. */
+float fl_source(float dt, int it);
+
+/* Coefficient precompute into the 10 FP32 volumes of precomp.h:1-12, in the
+ * order ch1dxx ch1dyy ch1dzz ch1dxy ch1dyz ch1dxz v2px v2pz v2sz v2pn —
+ * precomp.h:25-38 (trig in double, products in float) and :53-58. */
+void fl_precompute_f32(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                       const float *delta, const float *phi, const float *theta, float *const *coef);
+
+/* Same 10 coefficients evaluated entirely in FP64 from the FP32 model. */
+void fl_precompute_f64(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                       const float *delta, const float *phi, const float *theta,
+                       double *const *coef);
+
+/* Number of time steps ceil(tmax/dt) in float, main.c:101. */
+int fl_num_steps(float tmax, float dt);
+
+/* ---------------- seeded heterogeneous models (no reference counterpart) ----------------
+ * All generators fill the whole sx*sy*sz array, interior = [bord+absorb, bord+absorb+n)
+ * on each axis, then apply fl_random_boundary(boundary_seed, ...).
+ * vsv = vpz*sqrt(|eps-delta|/0.75) as main.c:218. */
+
+/* Linear vpz gradient vmin->vmax with depth over the interior, times (1 + sum of nblobs
+ * Gaussian blobs of amplitude in [-amp,amp], widths in [smin,smax] cells) clipped to
+ * [1-amp, 1+amp]; eps a smooth field in [0,eps_max]; delta = smooth*min(eps, delta_max)
+ * (so delta<=eps); theta smooth in [0,theta_max]; phi smooth in [0,phi_max). */
+typedef struct {
+    uint64_t seed;
+    float vmin, vmax;
+    int nblobs;
+    float blob_amp, blob_smin, blob_smax;
+    float eps_max, delta_max, theta_max, phi_max;
+    int nsmooth;            /* Gaussians per smooth field */
+    float smooth_smin, smooth_smax;
+    uint32_t boundary_seed;
+} fl_gradient_model_t;
+void fl_gradient_model_defaults(fl_gradient_model_t *m, uint64_t seed);
+int fl_model_gradient(const fl_gradient_model_t *m, int sx, int sy, int sz, int nx, int ny, int nz,
+                      int bord, int absorb, float *vpz, float *vsv, float *epsilon, float *delta,
+                      float *phi, float *theta);
+
+/* Dipping-layer model: nlayers planar layers with normals tilted up to max_dip_deg,
+ * vpz rising from vmin to vmax by layer, eps/delta piecewise per layer (delta<=eps),
+ * theta/phi smooth random fields. */
+typedef struct {
+    uint64_t seed;
+    int nlayers;
+    float vmin, vmax, max_dip_deg;
+    float eps_max, delta_max, theta_max, phi_max;
+    int nsmooth;
+    float smooth_smin, smooth_smax;
+    uint32_t boundary_seed;
+} fl_layered_model_t;
+void fl_layered_model_defaults(fl_layered_model_t *m, uint64_t seed);
+int fl_model_layered(const fl_layered_model_t *m, int sx, int sy, int sz, int nx, int ny, int nz,
+                     int bord, int absorb, float *vpz, float *vsv, float *epsilon, float *delta,
+                     float *phi, float *theta);
+
+/* dt from a Courant number on max(vpz*sqrt(1+2*eps)) over the array. */
+float fl_cfl_dt(int64_t n, const float *vpz, const float *epsilon, float h, float courant);
+
+/* Copy the window [x0,x0+wx)x[y0,y0+wy)x[z0,z0+wz) of a volume (sub-grid extraction
+ * for shots and oracle windows). */
+void fl_extract_window(const float *src, int sx, int sy, int sz, int x0, int y0, int z0, int wx,
+                       int wy, int wz, float *dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

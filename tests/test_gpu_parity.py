@@ -1,0 +1,228 @@
+"""GPU parity of the HIP back-end (through the C ABI) against the reference and the oracle.
+
+Bar: relative L2 <= 1e-5 on pc, qc and receiver traces against the reference C/OpenMP
+path (golden fixtures written by the reference program; the oracle, bitwise equal to it,
+elsewhere); bitwise equality where the arithmetic is the same by construction (tile
+shapes, graph vs call-by-call runs, the ISO reduction p == q, batched vs single shot).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+D = (12.5, 12.5, 12.5)
+TOL = 1e-5
+
+
+@pytest.fixture
+def drv(fl):
+    made = []
+
+    def make(**kw):
+        d = fl.Driver(**kw)
+        made.append(d)
+        return d
+
+    yield make
+    for d in made:
+        d.finalize()
+    fl.Driver(variant="fused", precision="fp32", sync=True, tiles=(8, 0)).finalize()
+
+
+def ref_model(fl, form, nx, absorb):
+    shape = fl.grid_shape(nx, nx, nx, absorb)
+    m = fl.model_homogeneous(form, shape)
+    fl.random_boundary(m, (nx, nx, nx), absorb)
+    return shape, m
+
+
+def hetero(fl, n3=(70, 45, 37), absorb=8, seed=11):
+    m = fl.model_gradient(n3, absorb, seed=seed, blob_smin=4.0, blob_smax=12.0, smooth_smin=6.0,
+                          smooth_smax=20.0)
+    dt = fl.cfl_dt(m, 12.5, 0.4)
+    return m, dt
+
+
+@pytest.mark.parametrize("variant", ["fused", "direct", "twopass"])
+@pytest.mark.parametrize("form", ["ISO", "VTI", "TTI"])
+@pytest.mark.parametrize("steps", [10, 30])
+def test_reference_fixture_parity(fl, drv, form, variant, steps):
+    ref_variant = "original" if variant == "direct" else "der1der1"
+    g = golden("ref_%s_%s_n8_a4_s%d.npz" % (form, ref_variant, steps))
+    shape, m = ref_model(fl, form, 8, 4)
+    d = drv(variant=variant)
+    d.initialize(shape, D, 0.001, m)
+    d.step_loop(steps, int(g["isource"]))
+    pc = d.update_pointers()
+    assert rel_l2(pc, g["pc"]) <= TOL
+    assert rel_l2(d.field("qc"), g["qc"]) <= TOL
+    assert rel_l2(d.field("pp"), g["pp"]) <= TOL
+
+
+def test_reference_fixture_parity_absorb16(fl, drv):
+    g = golden("ref_TTI_original_n24_a16_s10.npz")
+    shape, m = ref_model(fl, "TTI", 24, 16)
+    for variant in ("fused", "direct", "twopass"):
+        d = drv(variant=variant)
+        d.initialize(shape, D, 0.001, m)
+        d.step_loop(10, int(g["isource"]))
+        assert rel_l2(d.field("pc"), g["pc"]) <= TOL, variant
+        assert rel_l2(d.field("qc"), g["qc"]) <= TOL, variant
+        d.finalize()
+
+
+@pytest.mark.parametrize("variant", ["fused", "direct", "twopass"])
+def test_oracle_parity_heterogeneous_ragged(fl, oracle, drv, variant):
+    """Ragged grid (no extent a multiple of the tile), heterogeneous TTI model, off-centre
+    source, 160 steps, receiver traces every 4 steps."""
+    m, dt = hetero(fl)
+    shape = m[0].shape
+    isrc = fl.index(shape, shape[2] // 3, shape[1] // 2, 4 + 8 + 5)
+    rec = 4 + 8 + 5
+    st = 160
+    d = drv(variant=variant)
+    d.initialize(shape, D, dt, m)
+    d.record_traces(rec, 4, st // 4)
+    d.run(st, isrc, graph_chunk=0)
+    ov = "original" if variant == "direct" else "der1der1"
+    r = oracle.run(ov, shape, 4, D, dt, st, isrc, oracle.precompute(m), rec_iz=rec, rec_every=4)
+    assert rel_l2(d.field("pc"), r["pc"]) <= TOL
+    assert rel_l2(d.field("qc"), r["qc"]) <= TOL
+    assert rel_l2(d.traces(), r["traces"]) <= TOL
+    # and the other reference form (the two differ only by rounding)
+    r2 = oracle.run("der1der1" if ov == "original" else "original", shape, 4, D, dt, st, isrc,
+                    oracle.precompute(m))
+    assert rel_l2(d.field("pc"), r2["pc"]) <= TOL
+
+
+def test_fp64_oracle_error_reported(fl, oracle, drv):
+    m, dt = hetero(fl, (48, 40, 36), 6)
+    shape = m[0].shape
+    isrc = fl.index(shape, shape[2] // 2, shape[1] // 2, 4 + 6 + 4)
+    d = drv(variant="fused")
+    d.initialize(shape, D, dt, m)
+    d.run(200, isrc, graph_chunk=0)
+    r = oracle.run("der1der1", shape, 4, D, dt, 200, isrc, oracle.precompute(m, "f64"), fp="f64")
+    err = rel_l2(d.field("pc"), r["pc"])
+    print("fused FP32 vs FP64 oracle relL2(pc) = %.3e" % err)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("tiles", [(4, 16), (8, 0), (16, 24), (8, 5), (4, 200)])
+def test_tile_shapes_are_bitwise_identical(fl, drv, tiles):
+    m, dt = hetero(fl, (60, 33, 41), 6)
+    shape = m[0].shape
+    isrc = fl.index(shape, 20, 17, 15)
+    out = []
+    for t in [(8, 0), tiles]:
+        d = drv(variant="fused", tiles=t)
+        d.initialize(shape, D, dt, m)
+        d.run(40, isrc, graph_chunk=0)
+        out.append((d.field("pc"), d.field("qc")))
+        d.finalize()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_twopass_tiles_bitwise(fl, drv):
+    m, dt = hetero(fl, (40, 30, 28), 6)
+    shape = m[0].shape
+    isrc = fl.index(shape, 20, 17, 15)
+    out = []
+    for t in [(8, 0), (4, 9), (16, 64)]:
+        d = drv(variant="twopass", tiles=t)
+        d.initialize(shape, D, dt, m)
+        d.run(30, isrc, graph_chunk=0)
+        out.append(d.field("pc"))
+        d.finalize()
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+def test_graph_run_equals_call_by_call(fl, drv):
+    m, dt = hetero(fl, (50, 40, 30), 6)
+    shape = m[0].shape
+    isrc = fl.index(shape, 25, 20, 12)
+    d = drv(variant="fused")
+    d.initialize(shape, D, dt, m)
+    d.record_traces(14, 3, 30)
+    d.step_loop(77, isrc)
+    a = (d.field("pc"), d.field("qc"), d.field("pp"), d.traces())
+    d.finalize()
+    d = drv(variant="fused")
+    d.initialize(shape, D, dt, m)
+    d.record_traces(14, 3, 30)
+    d.run(40, isrc, graph_chunk=10)            # 4 graph replays
+    d.run(37, isrc, it0=40, graph_chunk=8)     # 4 replays + 5 direct steps
+    b = (d.field("pc"), d.field("qc"), d.field("pp"), d.traces())
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_iso_p_equals_q_and_halo_zero(fl, drv):
+    shape, m = ref_model(fl, "ISO", 40, 8)
+    for variant in ("fused", "direct", "twopass"):
+        d = drv(variant=variant)
+        d.initialize(shape, D, 0.001, m)
+        d.step_loop(60, fl.center_index(shape))
+        pc, qc, pp = d.field("pc"), d.field("qc"), d.field("pp")
+        assert np.array_equal(pc, qc), variant
+        assert np.abs(pc).max() > 0
+        for f in (pc, pp):
+            assert not f[:4].any() and not f[-4:].any() and not f[:, :4].any()
+            assert not f[:, -4:].any() and not f[:, :, :4].any() and not f[:, :, -4:].any()
+        d.finalize()
+
+
+def test_autotune_keeps_state_and_result(fl, drv, tmp_path):
+    m, dt = hetero(fl, (64, 40, 48), 6)
+    shape = m[0].shape
+    isrc = fl.index(shape, 30, 20, 14)
+    d = drv(variant="fused")
+    d.initialize(shape, D, dt, m)
+    d.run(20, isrc, graph_chunk=0)
+    before = d.field("pc")
+    cache = str(tmp_path / "tiles.txt")
+    ty, zt = d.autotune(2, cache)
+    assert ty in (4, 8, 16) and zt > 0
+    assert np.array_equal(d.field("pc"), before)
+    d.run(20, isrc, it0=20, graph_chunk=0)
+    after_tuned = d.field("pc")
+    d.finalize()
+    d = drv(variant="fused", tiles=(8, 0))
+    d.initialize(shape, D, dt, m)
+    d.run(40, isrc, graph_chunk=0)
+    assert np.array_equal(d.field("pc"), after_tuned)
+    assert open(cache).read().count("|") >= 4  # cache line written
+    d.finalize()
+    d = drv(variant="fused")
+    d.initialize(shape, D, dt, m)
+    assert d.autotune(2, cache) == (ty, zt)  # served from the cache
+
+
+def test_initialize_with_host_fields_and_update_pointers(fl, drv):
+    shape, m = ref_model(fl, "TTI", 16, 4)
+    rng = np.random.default_rng(0)
+    f = [np.zeros(shape, np.float32) for _ in range(4)]
+    inner = (slice(4, -4),) * 3
+    for a in f:
+        a[inner] = rng.standard_normal(a[inner].shape).astype(np.float32) * 1e-3
+    d = drv(variant="fused")
+    d.initialize(shape, D, 0.001, m, fields=f)
+    for k, name in enumerate(("pp", "pc", "qp", "qc")):
+        assert np.array_equal(d.field(name), f[k])
+    d.propagate(1)
+    assert np.array_equal(d.update_pointers(), d.field("pc"))
+
+
+def test_errors_are_reported_not_fatal(fl, drv):
+    shape, m = ref_model(fl, "TTI", 8, 4)
+    d = drv(variant="fused")
+    d.initialize(shape, D, 0.001, m)
+    with pytest.raises(fl.FletcherError):
+        d.insert_source(0, int(np.prod(shape)) + 5, 1.0)
+    with pytest.raises(fl.FletcherError):
+        fl.lib().DRIVER_Propagate(shape[2] + 1, shape[1], shape[0], 4, 12.5, 12.5, 12.5, 0.001, 1,
+                                  None, None, None, None)
+        d._status()
+    d.step_loop(2, fl.center_index(shape))  # still usable

@@ -18,7 +18,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfletcher_amd.so")
+# FL_LIBRARY selects another build of the same C ABI (A/B comparisons of kernel builds)
+LIB_PATH = os.environ.get("FL_LIBRARY") or os.path.join(_HERE, "libfletcher_amd.so")
 CLI_PATH = os.path.join(_HERE, "bin", "fletcher")
 
 BORD = 4  # stencil halo (main.c:21)
@@ -78,6 +79,7 @@ def lib():
         "fl_region_begin": (i, []),
         "fl_region_end": (i, [C.POINTER(d)]),
         "fl_device_name": (i, [C.c_char_p, i]),
+        "fl_get_derivative_volume": (i, [i, _f32p]),
         "fl_device_bytes": (ll, []),
         "fl_bytes_per_cell": (i, []),
         "fl_batch_create": (i, [i, vp, i, f, f, f, f, i]),
@@ -333,6 +335,12 @@ class Driver:
         for it in range(it0 + 1, it0 + st + 1):
             self.insert_source(it - 1, isource, source(self.dt, it - 1))
             self.propagate(it)
+
+    def derivative_volume(self, k):
+        """Two-pass first-derivative volume k (0 pDx, 1 pDy, 2 qDx, 3 qDy) of the last step."""
+        out = np.empty(self.shape, np.float32)
+        _check(self.L.fl_get_derivative_volume(k, out))
+        return out
 
     def set_variant(self, v):
         _check(self.L.fl_set_variant(VARIANT[v]))

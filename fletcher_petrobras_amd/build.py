@@ -27,6 +27,9 @@ HOST_SRC = ["fl_model.c", "fl_synth.c", "fl_run.c"]
 HIP_SRC = ["fl_driver.hip"]
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
+    # FMAs are written explicitly in the step kernels: no implicit contraction, so every
+    # copy of an expression rounds the same way
+    "-ffp-contract=off",
     # keep packed FP32 (v_pk_*_f32) out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
@@ -63,6 +66,9 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     _run(["gcc", "-O2", "-std=gnu11", "-I" + INC, os.path.join(PKG, "host", "fl_main.c"),
           "-L" + PKG, "-lfletcher_amd", "-Wl,-rpath,$ORIGIN/..", "-o", BIN], quiet=not verbose)
+    # HBM copy calibration used by the roofline / PMC profiling
+    _run([HIPCC] + HIP_FLAGS + [os.path.join(PKG, "csrc", "fl_stream.hip"), "-o",
+                                os.path.join(PKG, "bin", "fl_stream")], quiet=not verbose)
     return LIB
 
 

@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "fl_zmarch.h"
+#include "fl_zmarch_sf.h"
 #include "fletcher/driver.h"
 #include "fletcher/fl_hip.h"
 #include "fletcher/fl_model.h"
@@ -107,6 +107,11 @@ static Tiles g_tiles;
 static bool g_timing = false;
 static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_ev_used, g_ev_free;
 static hipEvent_t g_reg0 = nullptr, g_reg1 = nullptr;
+// step kernel family: 1 split-field (one field per half wave, 32-column tiles),
+// 0 one lane per cell (64-column tiles); FL_KERNEL=cell|sf selects it
+static int g_sf = 1;
+
+static int tile_x() { return g_sf ? 32 : 64; }
 
 static int ensure_stream()
 {
@@ -115,7 +120,10 @@ static int ensure_stream()
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev < 1) return fail(FL_ERR_HIP, "no HIP device visible");
     HIPCHK(hipSetDevice(0));
-    HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    // default flags: the stream also orders after legacy null-stream work (hipMemcpy etc.)
+    HIPCHK(hipStreamCreate(&g_stream));
+    const char *k = getenv("FL_KERNEL");
+    if (k && strcmp(k, "cell") == 0) g_sf = 0;
     return FL_OK;
 }
 
@@ -198,10 +206,10 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
 // ------------------------------------------------------------------ launch plumbing
 static int default_zt(int sx, int sy, int sz, int bord, int ty)
 {
-    const int ntx = (sx - 2 * bord + 63) / 64, nty = (sy - 2 * bord + ty - 1) / ty;
+    const int ntx = (sx - bord + tile_x() - 1) / tile_x(), nty = (sy - 2 * bord + ty - 1) / ty;
     const int nzu = sz - 2 * bord;
     const long long per_plane = (long long)ntx * nty;
-    const long long target = 2048;  // ~4 resident waves of 2 blocks per CU on 256 CUs
+    const long long target = 4096;  // measured best on 256 CUs: ~8 z chunks at 384^3
     int nzc = (int)std::max<long long>(1, (target + per_plane - 1) / per_plane);
     int zt = (nzu + nzc - 1) / nzc;
     zt = std::max(zt, 16);
@@ -211,7 +219,7 @@ static int default_zt(int sx, int sy, int sz, int bord, int ty)
 static void fill_geom(StepArgs &a, int sx, int sy, int sz, int bord, int ty, int zt)
 {
     a.sx = sx; a.sy = sy; a.sz = sz; a.bord = bord;
-    a.ntx = (sx - 2 * bord + 63) / 64;
+    a.ntx = (sx - bord + tile_x() - 1) / tile_x();  // tiles [0,TX), [TX,2TX) ... up to sx-bord
     a.nty = (sy - 2 * bord + ty - 1) / ty;
     a.zt = zt;
     const int nzc = (sz - 2 * bord + zt - 1) / zt;
@@ -222,7 +230,10 @@ template <int TY, int CM, bool TP>
 static void launch_zm(const StepArgs &a, hipStream_t s)
 {
     const int g = (a.nblk + 7) / 8 * 8;
-    hipLaunchKernelGGL((zmarch<TY, CM, TP>), dim3(g), dim3(64, TY), 0, s, a);
+    if (g_sf)
+        hipLaunchKernelGGL((zmarch_sf<TY, CM, TP>), dim3(g), dim3(64, TY), 0, s, a);
+    else
+        hipLaunchKernelGGL((zmarch<TY, CM, TP>), dim3(g), dim3(64, TY), 0, s, a);
 }
 
 static int launch_zmarch(const StepArgs &a, int ty, int cm, bool tp, hipStream_t s)
@@ -281,7 +292,7 @@ static int ensure_dvol()
     if (D.dvol) return FL_OK;
     int rc = dev_alloc((void **)&D.dvol, (size_t)4 * D.n * 4, &D.bytes);
     if (rc) return rc;
-    HIPCHK(hipMemset(D.dvol, 0, (size_t)4 * D.n * 4));
+    HIPCHK(hipMemsetAsync(D.dvol, 0, (size_t)4 * D.n * 4, g_stream));
     return FL_OK;
 }
 
@@ -385,7 +396,7 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
     const float *host[4] = {pp, pc, qp, qc};
     for (int k = 0; k < 4; k++) {
         hipError_t e = host[k] ? hipMemcpy(D.f[k], host[k], (size_t)n * 4, hipMemcpyHostToDevice)
-                               : hipMemset(D.f[k], 0, (size_t)n * 4);
+                               : hipMemsetAsync(D.f[k], 0, (size_t)n * 4, g_stream);
         if (e != hipSuccess) { free_device(); return fail(FL_ERR_HIP, "field upload: %s", hipGetErrorString(e)); }
     }
     const size_t cbytes = (size_t)coef_count(D.cm) * n * coef_elem(D.cm);
@@ -537,6 +548,15 @@ extern "C" int fl_set_field(int which, const float *host)
     return FL_OK;
 }
 
+extern "C" int fl_get_derivative_volume(int k, float *host)
+{
+    if (!D.init || !D.dvol) return fail(FL_ERR_STATE, "no two-pass step has run");
+    if (k < 0 || k > 3 || !host) return fail(FL_ERR_ARG, "bad derivative volume %d", k);
+    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpy(host, D.dvol + k * D.n, (size_t)D.n * 4, hipMemcpyDeviceToHost));
+    return FL_OK;
+}
+
 extern "C" int fl_record_traces(int iz, int every, int max_traces)
 {
     if (!D.init) return fail(FL_ERR_STATE, "no initialised grid");
@@ -553,7 +573,7 @@ extern "C" int fl_record_traces(int iz, int every, int max_traces)
     const size_t bytes = (size_t)max_traces * (D.sx - 2 * D.bord) * (D.sy - 2 * D.bord) * 4;
     int rc = dev_alloc((void **)&D.trace, bytes, &D.bytes);
     if (rc) return rc;
-    HIPCHK(hipMemset(D.trace, 0, bytes));
+    HIPCHK(hipMemsetAsync(D.trace, 0, bytes, g_stream));
     D.rec_iz = iz; D.rec_every = every; D.rec_slots = max_traces;
     return FL_OK;
 }
@@ -758,11 +778,14 @@ static std::string tune_key(const char *kind, int sx, int sy, int sz, int cm)
 
 static std::vector<int> zt_candidates(int nzu)
 {
+    // fixed depths plus even splits of the column into 2..16 chunks
     std::vector<int> c = {16, 24, 32, 48, 64, 96, 128, 192};
+    for (int k : {2, 4, 6, 8, 12, 16}) c.push_back((nzu + k - 1) / k);
     std::vector<int> r;
     for (int z : c)
-        if (z < nzu) r.push_back(z);
+        if (z >= 8 && z < nzu && std::find(r.begin(), r.end(), z) == r.end()) r.push_back(z);
     r.push_back(nzu);
+    std::sort(r.begin(), r.end());
     return r;
 }
 

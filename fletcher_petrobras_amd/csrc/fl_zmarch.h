@@ -30,42 +30,64 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base, u
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
                                              0x00020000);
 }
+// Cache-policy bits of the once-per-step streams (coefficients, pp/qp): 0 default,
+// 2 = nt (streaming: not kept in L2, which then holds the pc/qc halos).
+#ifndef FL_STREAM_AUX
+#define FL_STREAM_AUX 0
+#endif
 __device__ __forceinline__ float bld_f32(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, FL_STREAM_AUX));
 }
 __device__ __forceinline__ uint16_t bld_u16(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, FL_STREAM_AUX);
 }
 __device__ __forceinline__ void bst_f32(__amdgpu_buffer_rsrc_t r, uint32_t off, float v)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, FL_STREAM_AUX);
 }
+
+// Raw coefficient words of one cell as loaded (decoded only where they are used, so
+// the loads stay in flight across barriers).
+struct CoefRaw {
+    uint32_t w[10];
+};
 
 // Coefficients of one cell: `plane` = coefficient volume 0 at the cell's z plane,
 // volumes `vb` bytes apart, each plane `pb` bytes; `off` = lane byte offset.
 template <int CM>
-__device__ __forceinline__ void load_coef_b(const char *plane, long long vb, uint32_t pb, uint32_t off,
-                                            Coef &k)
+__device__ __forceinline__ void load_coef_raw(const char *plane, long long vb, uint32_t pb, uint32_t off,
+                                              CoefRaw &r)
+{
+    constexpr int NC = (CM == CM_F16R) ? 6 : 10;
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        if constexpr (CM == CM_F32)
+            r.w[j] = __builtin_bit_cast(uint32_t, bld_f32(plane_rsrc(plane + j * vb, pb), off));
+        else
+            r.w[j] = bld_u16(plane_rsrc(plane + j * vb, pb), off);
+    }
+}
+
+template <int CM>
+__device__ __forceinline__ void decode_coef(const CoefRaw &r, Coef &k)
 {
     if constexpr (CM == CM_F32) {
 #pragma unroll
-        for (int j = 0; j < 10; j++) k.c[j] = bld_f32(plane_rsrc(plane + j * vb, pb), off);
+        for (int j = 0; j < 10; j++) k.c[j] = __builtin_bit_cast(float, r.w[j]);
     } else if constexpr (CM == CM_BF16 || CM == CM_F16) {
 #pragma unroll
-        for (int j = 0; j < 10; j++) {
-            const uint16_t h = bld_u16(plane_rsrc(plane + j * vb, pb), off);
-            k.c[j] = (CM == CM_BF16) ? bf16_to_f32(h) : f16_to_f32(h);
-        }
-    } else {  // CM_F16R: recompute from vpz*dt, vsv*dt, eps, delta, theta, phi-pi
-        float r[6];
+        for (int j = 0; j < 10; j++)
+            k.c[j] = (CM == CM_BF16) ? bf16_to_f32((uint16_t)r.w[j]) : f16_to_f32((uint16_t)r.w[j]);
+    } else {  // CM_F16R: rebuild from vpz*dt, vsv*dt, eps, delta, theta, phi-pi
+        float v[6];
 #pragma unroll
-        for (int j = 0; j < 6; j++) r[j] = f16_to_f32(bld_u16(plane_rsrc(plane + j * vb, pb), off));
-        const float w2 = r[0] * r[0];
+        for (int j = 0; j < 6; j++) v[j] = f16_to_f32((uint16_t)r.w[j]);
+        const float w2 = v[0] * v[0];
         float st, ct, spc, cpc;
-        __sincosf(r[4], &st, &ct);
-        __sincosf(r[5], &spc, &cpc);
+        __sincosf(v[4], &st, &ct);
+        __sincosf(v[5], &spc, &cpc);
         const float sp = -spc, cp = -cpc;  // phi = phic + pi
         const float st2 = st * st;
         const float s2t = 2.0f * st * ct;
@@ -76,11 +98,20 @@ __device__ __forceinline__ void load_coef_b(const char *plane, long long vb, uin
         k.c[3] = st2 * s2p;
         k.c[4] = s2t * sp;
         k.c[5] = s2t * cp;
-        k.c[6] = w2 * (1.0f + 2.0f * r[2]);
+        k.c[6] = w2 * (1.0f + 2.0f * v[2]);
         k.c[7] = w2;
-        k.c[8] = r[1] * r[1];
-        k.c[9] = w2 * (1.0f + 2.0f * r[3]);
+        k.c[8] = v[1] * v[1];
+        k.c[9] = w2 * (1.0f + 2.0f * v[3]);
     }
+}
+
+template <int CM>
+__device__ __forceinline__ void load_coef_b(const char *plane, long long vb, uint32_t pb, uint32_t off,
+                                            Coef &k)
+{
+    CoefRaw r;
+    load_coef_raw<CM>(plane, vb, pb, off, r);
+    decode_coef<CM>(r, k);
 }
 
 template <bool B>
@@ -100,7 +131,9 @@ __device__ __forceinline__ void zmarch_tile(const StepArgs &a, const StepCtl &ct
     constexpr int TX = 64, HX = TX + 8, HY = TY + 8, NT = 64 * TY;
     constexpr int NLD = (HX * HY + NT - 1) / NT;  // halo-tile loads per thread per field
     constexpr int ES = (CM == CM_F32) ? 4 : 2;    // coefficient element bytes
-    const int x0 = a.bord + tix * TX, y0 = a.bord + tiy * TY;
+    // tiles start at absolute multiples of TX so every row access is line-aligned
+    // (for row pitches that are); lanes left of bord or right of sx-bord only load
+    const int x0 = tix * TX, y0 = a.bord + tiy * TY;
     const int zlo = a.bord + zc * a.zt;
     const int zhi = min(zlo + a.zt, a.sz - a.bord);
     const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
@@ -122,7 +155,7 @@ __device__ __forceinline__ void zmarch_tile(const StepArgs &a, const StepCtl &ct
         const int e = tid + k * NT;
         const int r = e / HX, c = e - r * HX;
         const int gy = y0 - 4 + r, gx = x0 - 4 + c;
-        hok[k] = (e < HX * HY) && gx < a.sx && gy < a.sy;
+        hok[k] = (e < HX * HY) && gx >= 0 && gx < a.sx && gy < a.sy;
         hoff[k] = hok[k] ? (uint32_t)(gy * a.sx + gx) * 4u : 0u;
     }
     float pre_p[NLD], pre_q[NLD];

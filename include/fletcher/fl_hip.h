@@ -80,6 +80,10 @@ int fl_kernel_times(double *total_ms, int *count);
 int fl_region_begin(void);
 int fl_region_end(double *ms);
 
+/* Diagnostics: the two-pass variant's first-derivative volume k (0 pDx, 1 pDy, 2 qDx,
+ * 3 qDy; der1der1/OpenMP/openmp_propagate.c:24-76) of the last step, n floats. */
+int fl_get_derivative_volume(int k, float *host);
+
 int fl_device_name(char *buf, int len);
 long long fl_device_bytes(void);   /* device memory held by the back-end */
 /* Algorithmic HBM bytes per updated cell per step of the current configuration. */

@@ -1,6 +1,7 @@
-"""Headline benchmark: MSamples/s of the FP32 TTI drop-in at 344^3 on one MI355X.
+"""Headline benchmark: MSamples/s of the FP32 TTI drop-in at 344^3 on one MI355X, with the
+mixed-precision and batched forms measured beside it.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--quick]
 
 Workload (synthetic, seeded): interior 344^3 with a 16-cell absorbing zone and the
 4-cell halo (array 384^3), dx=dy=dz=12.5 m; vpz rising 1500->4500 m/s with depth
@@ -17,6 +18,14 @@ read and written, 10 FP32 coefficients) over its mean duration from HIP events r
 around every launch on the library's stream during the timed region.
 cpu_baseline: the reference C/OpenMP program (oracle/_ref, built from the reference
 sources) timed on a few steps of the same grid on the host cores.
+
+extras (skipped with --quick), each timed over its whole run with inputs resident:
+  twopass   the der1der1 two-pass variant on the same grid (104 algorithmic B/sample)
+  mixed     BF16 / FP16 coefficients and the FP16 in-kernel recompute on the same grid,
+            receivers on plane z=28 every 4 steps, 50-step graph chunks
+  batched   32 shots on uneven (128..256)^2 x 320 apertures cut from one seeded
+            512x512x320 model (absorb 16), 800 steps in 50-step graphs, autotuned
+            tiles, FP32 and BF16 coefficients; receivers every 4 steps
 """
 import argparse
 import json
@@ -28,6 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+D = (12.5, 12.5, 12.5)
 
 
 def log(msg):
@@ -53,7 +63,7 @@ def cpu_baseline(model, shape, dt, isrc, steps, threads):
         return None
     R = O.Ref("original")
     R.set_threads(threads)
-    R.setup(shape, 4, (12.5, 12.5, 12.5), dt, model)
+    R.setup(shape, 4, D, dt, model)
     R.steps(0, 1, isrc)  # first touch
     wall = R.steps(1, steps, isrc)
     samples = (shape[0] - 8) * (shape[1] - 8) * (shape[2] - 8)
@@ -61,57 +71,33 @@ def cpu_baseline(model, shape, dt, isrc, steps, threads):
     return {"value": round(1e-6 * samples * steps / wall, 2), "unit": "MSamples/s", "cores": threads,
             "kind": "reference",
             "sample": "%d steps of the reference original/ OpenMP Propagate (direct DerCross) on the "
-                      "same 384^3 grid and model, Propagate+swap wall time as model.c:82-88" % steps}
+                      "same %dx%dx%d grid and model, Propagate+swap wall time as model.c:82-88"
+                      % (steps, shape[2], shape[1], shape[0])}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=344)
-    ap.add_argument("--absorb", type=int, default=16)
-    ap.add_argument("--variant", default="fused", choices=["fused", "direct", "twopass"])
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp16r"])
-    ap.add_argument("--tiles", default="auto", help="ty,zt or 'auto' (autotuner)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: the Fletcher hot path here is single-GPU "
-              "(multi-GPU halo exchange is out of scope); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-    import numpy as np
-    import fletcher_petrobras_amd as F
+def tiles_for(d, spec):
+    if spec == "auto":
+        return d.autotune(3)
+    ty, zt = (int(v) for v in spec.split(","))
+    d.set_tiles(ty, zt)
+    return ty, zt
 
-    n = args.nx
-    t0 = time.time()
-    model = F.model_gradient((n, n, n), args.absorb, seed=344)
-    shape = model[0].shape
-    dt = F.cfl_dt(model, 12.5, 0.4)
-    isrc = F.index(shape, shape[2] // 2, shape[1] // 2, 4 + args.absorb + 8)
+
+def dropin(F, np, model, shape, dt, isrc, args):
+    """The reference loop through the drop-in entry points, per-launch HIP events."""
     samples = (shape[0] - 8) * (shape[1] - 8) * (shape[2] - 8)
-    log("model %s built in %.1f s, dt=%.6g" % (shape, time.time() - t0, dt))
-
     d = F.Driver(variant=args.variant, precision=args.precision, sync=False)
     t0 = time.time()
-    d.initialize(shape, (12.5, 12.5, 12.5), dt, model)
+    d.initialize(shape, D, dt, model)
     log("initialize (precompute + upload) %.1f s, device %s" % (time.time() - t0, F.device_name()))
-    if args.tiles == "auto":
-        ty, zt = d.autotune(3)
-    else:
-        ty, zt = (int(v) for v in args.tiles.split(","))
-        d.set_tiles(ty, zt)
+    ty, zt = tiles_for(d, args.tiles)
     log("tiles ty=%d zt=%d" % (ty, zt))
     bpc = d.bytes_per_cell()
-
-    # warm-up steps it = 1..W
     for it in range(1, args.warmup + 1):
         d.insert_source(it - 1, isrc, F.source(dt, it - 1))
         d.propagate(it)
     d.synchronize()
     d.kernel_times()  # clear
-    # timed region: steps W+1 .. W+K, per-launch HIP events on the library stream
     src = F.sources(dt, args.steps, args.warmup)
     d.set_timing(True)
     d.synchronize()
@@ -128,15 +114,129 @@ def main():
     pc = d.field("pc")
     finite = bool(np.isfinite(pc).all()) and float(np.abs(pc).max()) > 0
     d.finalize()
-    sec = region_ms * 1e-3
-    value = 1e-6 * samples * args.steps / sec
     avg_ms = kms / max(kcount, 1)
-    achieved = bpc * samples / (avg_ms * 1e-3) / 1e9
-    log("region %.1f ms (host wall %.1f ms), kernel avg %.4f ms over %d, %.0f GB/s, finite=%s"
-        % (region_ms, wall * 1e3, avg_ms, kcount, achieved, finite))
-    if not finite:
+    r = {"value": 1e-6 * samples * args.steps / (region_ms * 1e-3), "region_ms": region_ms,
+         "avg_kernel_ms": avg_ms, "launches": kcount, "bpc": bpc, "tiles": (ty, zt),
+         "achieved": bpc * samples / (avg_ms * 1e-3) / 1e9, "finite": finite}
+    log("drop-in %s/%s: region %.1f ms (host wall %.1f ms), kernel avg %.4f ms over %d, %.0f GB/s, "
+        "%.0f MS/s, finite=%s" % (args.variant, args.precision, region_ms, wall * 1e3, avg_ms, kcount,
+                                  r["achieved"], r["value"], finite))
+    return r
+
+
+def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, tiles):
+    """steps time steps in 50-step graph chunks, receivers every 4 steps; whole-run timing."""
+    samples = (shape[0] - 8) * (shape[1] - 8) * (shape[2] - 8)
+    d = F.Driver(variant=variant, precision=precision, sync=False)
+    d.initialize(shape, D, dt, model)
+    ty, zt = tiles_for(d, tiles)
+    bpc = d.bytes_per_cell()
+    d.record_traces(rec_iz, 4, (steps + 3) // 4)
+    src = F.sources(dt, steps)
+    d.run(2, isrc, graph_chunk=0, src=src[:2])  # warm-up (steps 1-2)
+    d.synchronize()
+    d.region_begin()
+    d.run(steps - 2, isrc, it0=2, graph_chunk=50, src=src[2:])
+    region_ms = d.region_end()
+    tr = d.traces()
+    pc = d.field("pc")
+    d.finalize()
+    finite = bool(np.isfinite(pc).all() and np.isfinite(tr).all()) and float(np.abs(tr).max()) > 0
+    value = 1e-6 * samples * (steps - 2) / (region_ms * 1e-3)
+    r = {"value": round(value, 1), "ms_per_step": round(region_ms / (steps - 2), 4),
+         "bytes_per_sample": bpc, "achieved_GBs": round(bpc * value * 1e-3, 1),
+         "frac": round(bpc * value * 1e-3 / HBM_PEAK_GBS, 4), "tiles": [ty, zt], "finite": finite}
+    log("%s/%s graph run: %s" % (variant, precision, r))
+    return r
+
+
+def batched(F, np, precisions, steps, nshots):
+    """Batched shots on apertures cut from one 512x512x320 parent model."""
+    from fletcher_petrobras_amd import shots as S
+    absorb = 16
+    t0 = time.time()
+    parent = F.model_gradient((512, 512, 320), absorb, seed=512)
+    interior = S.parent_interior(parent, absorb)
+    sh = S.make_shots(interior, nshots, absorb, 128, 256, src_depth=4 + absorb + 8,
+                      rec_depth=4 + absorb + 8, seed=32)
+    del parent, interior
+    dt = min(F.cfl_dt(s["model"], 12.5, 0.4) for s in sh)
+    samples = sum((s["shape"][0] - 8) * (s["shape"][1] - 8) * (s["shape"][2] - 8) for s in sh)
+    log("batched: %d shots, %.3g samples/step, dt=%.6g, setup %.1f s"
+        % (len(sh), samples, dt, time.time() - t0))
+    out = {"shots": len(sh), "samples_per_step": samples, "steps": steps, "graph_chunk": 50,
+           "dt": dt}
+    src = F.sources(dt, steps)
+    for prec in precisions:
+        t0 = time.time()
+        b = F.Batch(sh, D, dt, precision=prec)
+        try:
+            b.autotune(4)
+            b.run(steps, graph_chunk=50, rec_every=4, max_traces=(steps + 3) // 4, src=src)
+            ms = b.last_ms()
+            tr = b.traces(0)
+            finite = bool(np.isfinite(tr).all()) and float(np.abs(tr).max()) > 0
+        finally:
+            b.destroy()
+        value = 1e-6 * samples * steps / (ms * 1e-3)
+        out[prec] = {"value": round(value, 1), "ms_per_step": round(ms / steps, 4), "finite": finite}
+        log("batched %s: %.0f MS/s (%.1f ms for %d steps; %.1f s incl. upload and tuning)"
+            % (prec, value, ms, steps, time.time() - t0))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=344)
+    ap.add_argument("--absorb", type=int, default=16)
+    ap.add_argument("--variant", default="fused", choices=["fused", "direct", "twopass"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp16r"])
+    ap.add_argument("--tiles", default="auto", help="ty,zt or 'auto' (autotuner)")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline line only, no extras")
+    ap.add_argument("--shots", type=int, default=32)
+    ap.add_argument("--batch-steps", type=int, default=800)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the Fletcher hot path here is single-GPU "
+              "(multi-GPU halo exchange is out of scope); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import fletcher_petrobras_amd as F
+
+    n = args.nx
+    t0 = time.time()
+    model = F.model_gradient((n, n, n), args.absorb, seed=344)
+    shape = model[0].shape
+    dt = F.cfl_dt(model, 12.5, 0.4)
+    rec_iz = 4 + args.absorb + 8
+    isrc = F.index(shape, shape[2] // 2, shape[1] // 2, rec_iz)
+    samples = (shape[0] - 8) * (shape[1] - 8) * (shape[2] - 8)
+    log("model %s built in %.1f s, dt=%.6g" % (shape, time.time() - t0, dt))
+
+    main_r = dropin(F, np, model, shape, dt, isrc, args)
+    if not main_r["finite"]:
         print("bench: the wavefield is not finite; refusing to report", file=sys.stderr)
         sys.exit(1)
+    value = main_r["value"]
+
+    extras = None
+    if not args.quick:
+        extras = {"twopass": graph_run(F, np, model, shape, dt, isrc, "twopass", "fp32", args.steps,
+                                       rec_iz, "auto"),
+                  "mixed": {}}
+        for prec in ("bf16", "fp16", "fp16r"):
+            r = graph_run(F, np, model, shape, dt, isrc, "fused", prec, args.steps, rec_iz, "auto")
+            r["vs_fp32_dropin"] = round(r["value"] / value, 3)
+            extras["mixed"][prec] = r
+        bt = batched(F, np, ("fp32", "bf16"), args.batch_steps, args.shots)
+        for prec in ("fp32", "bf16"):
+            bt[prec]["vs_fp32_dropin"] = round(bt[prec]["value"] / value, 3)
+        extras["batched"] = bt
 
     cpu = None
     if not args.no_cpu:
@@ -145,6 +245,9 @@ def main():
         cpu = cpu_baseline(model, shape, dt, isrc, args.cpu_steps, threads)
         log("cpu baseline %s (%.1f s)" % (cpu, time.time() - t0))
 
+    ty, zt = main_r["tiles"]
+    bpc = main_r["bpc"]
+    achieved = main_r["achieved"]
     tag = "%s_%s_%d" % (args.variant, args.precision, n)
     line = {
         "metric": "MSamples/s and % HBM peak, TTI 344^3: FP32 drop-in vs mixed-prec batched",
@@ -153,7 +256,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(region_ms / args.steps, 4),
+        "ms_per_step": round(main_r["region_ms"] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -161,7 +264,7 @@ def main():
         "data": "synthetic seeded heterogeneous TTI model (vpz gradient + blobs, smooth eps/delta/theta/phi, "
                 "reference random boundary)",
         "config": {"workload": "TTI %d^3 interior, absorb %d, array %dx%dx%d, dx=12.5 m, dt=%.6g s, "
-                               "%s variant, %s coefficients, tiles %dx64x%d"
+                               "%s variant, %s coefficients, tiles 32x%dx%d"
                                % (n, args.absorb, shape[2], shape[1], shape[0], dt, args.variant,
                                   args.precision, ty, zt),
                    "samples_per_step": samples, "bytes_per_sample": bpc},
@@ -170,6 +273,8 @@ def main():
         "pct_hbm_peak": round(100.0 * achieved / HBM_PEAK_GBS, 2),
         "cpu_baseline": cpu,
     }
+    if extras is not None:
+        line["extras"] = extras
     print(json.dumps(line), flush=True)
 
 

@@ -105,6 +105,10 @@ def lib():
         "fl_model_layered": (i, [vp] + [i] * 8 + [_f32p] * 6),
         "fl_layered_model_defaults": (None, [vp, C.c_uint64]),
         "fl_cfl_dt": (f, [C.c_int64, _f32p, _f32p, f, f]),
+        "fl_f64_to_f16": (C.c_uint16, [d]),
+        "fl_f64_to_bf16": (C.c_uint16, [d]),
+        "fl_pack_coefs16": (None, [C.c_int64] + [_f32p] * 6 + [d, i, C.POINTER(vp)]),
+        "fl_pack_recompute16": (None, [C.c_int64] + [_f32p] * 6 + [d, C.POINTER(vp)]),
         "fl_extract_window": (None, [_f32p] + [i] * 9 + [_f32p]),
         "fl_rsf_write_field": (i, [C.c_char_p] + [i] * 9 + [f, f, f, _f32p]),
         "fl_rsf_write_traces": (i, [C.c_char_p, i, i, i, f, f, f, _f32p]),

@@ -53,29 +53,6 @@ extern "C" void fl_clear_error(void)
 }
 
 // ------------------------------------------------------------------ helpers
-// FP64 -> BF16 with one round-to-nearest-even of the 8-bit significand (no FP32 detour).
-static uint16_t f64_to_bf16(double v)
-{
-    double r = v;
-    if (v != 0.0 && std::isfinite(v)) {
-        int e;
-        const double m = std::frexp(v, &e);  // v = m 2^e, 0.5 <= |m| < 1
-        r = std::ldexp(std::nearbyint(std::ldexp(m, 8)), e - 8);
-    }
-    const float f = (float)r;  // exact: r has at most 8 significant bits
-    uint32_t u;
-    memcpy(&u, &f, 4);
-    return (uint16_t)(u >> 16);
-}
-
-static uint16_t f64_to_f16(double v)
-{
-    _Float16 h = (_Float16)v;  // correctly rounded double -> half
-    uint16_t b;
-    memcpy(&b, &h, 2);
-    return b;
-}
-
 // ------------------------------------------------------------------ state
 struct Tiles {
     int ty = 8, zt = 0;
@@ -156,9 +133,7 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
                         long long n, long long sxy, int sz)
 {
     const int slab = std::max(1, (int)std::min<long long>(sz, (64ll << 20) / (sxy * 4)));
-    const int nc = coef_count(cm);
     std::vector<float> f32;
-    std::vector<double> f64;
     std::vector<uint16_t> h16;
     for (int z0 = 0; z0 < sz; z0 += slab) {
         const int z1 = std::min(sz, z0 + slab);
@@ -170,34 +145,18 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
             fl_precompute_f32(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, c);
             for (int k = 0; k < 10; k++)
                 HIPCHK(hipMemcpy((float *)dst + k * n + off, c[k], m * 4, hipMemcpyHostToDevice));
-        } else if (cm == CM_BF16 || cm == CM_F16) {
-            f64.resize(10 * m);
-            h16.resize(10 * m);
-            double *c[10];
-            for (int k = 0; k < 10; k++) c[k] = f64.data() + k * m;
-            fl_precompute_f64(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, c);
-            const double dt2 = (double)dt * dt;
-            for (long long i = 0; i < 10 * m; i++) {
-                const double v = (i >= 6 * m) ? f64[i] * dt2 : f64[i];
-                h16[i] = cm == CM_BF16 ? f64_to_bf16(v) : f64_to_f16(v);
-            }
-            for (int k = 0; k < 10; k++)
-                HIPCHK(hipMemcpy((uint16_t *)dst + k * n + off, h16.data() + k * m, m * 2,
-                                 hipMemcpyHostToDevice));
         } else {
-            h16.resize(6 * m);
-            const double pi = 3.14159265358979323846;
-            for (long long i = 0; i < m; i++) {
-                h16[i] = f64_to_f16((double)vpz[off + i] * dt);
-                h16[m + i] = f64_to_f16((double)vsv[off + i] * dt);
-                h16[2 * m + i] = f64_to_f16((double)eps[off + i]);
-                h16[3 * m + i] = f64_to_f16((double)del[off + i]);
-                h16[4 * m + i] = f64_to_f16((double)theta[off + i]);
-                h16[5 * m + i] = f64_to_f16((double)phi[off + i] - pi);
-            }
-            for (int k = 0; k < nc; k++)
-                HIPCHK(hipMemcpy((uint16_t *)dst + k * n + off, h16.data() + k * m, m * 2,
-                                 hipMemcpyHostToDevice));
+            const int nk = coef_count(cm);
+            h16.resize((size_t)nk * m);
+            uint16_t *c[10];
+            for (int k = 0; k < nk; k++) c[k] = h16.data() + k * m;
+            if (cm == CM_F16R)
+                fl_pack_recompute16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt, c);
+            else
+                fl_pack_coefs16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
+                                cm == CM_BF16, c);
+            for (int k = 0; k < nk; k++)
+                HIPCHK(hipMemcpy((uint16_t *)dst + k * n + off, c[k], m * 2, hipMemcpyHostToDevice));
         }
     }
     return FL_OK;

@@ -231,3 +231,76 @@ void fl_extract_window(const float *src, int sx, int sy, int sz, int x0, int y0,
             memcpy(dst + ((int64_t)iz * wy + iy) * wx,
                    src + (((int64_t)(z0 + iz)) * sy + (y0 + iy)) * sx + x0, sizeof(float) * wx);
 }
+
+/* ------------------------------------------------------------------ 16-bit packs */
+
+uint16_t fl_f64_to_f16(double v)
+{
+    const uint16_t sign = signbit(v) ? 0x8000 : 0;
+    if (isnan(v)) return 0x7e00;
+    const double a = fabs(v);
+    if (a >= 65520.0) return sign | 0x7c00; /* rounds to infinity */
+    int e;
+    const double m = frexp(a, &e); /* a = m 2^e, 0.5 <= m < 1 */
+    if (a != 0.0 && e - 1 >= -14) {
+        double q = nearbyint(ldexp(m, 11)); /* 11-bit significand, ties to even */
+        int E = e - 1;
+        if (q >= 2048.0) {
+            q = 1024.0;
+            E += 1;
+        }
+        return (uint16_t)(sign | ((E + 15) << 10) | ((int)q - 1024));
+    }
+    /* subnormal range: multiples of 2^-24 (1024 * 2^-24 encodes the smallest normal) */
+    return (uint16_t)(sign | (uint16_t)nearbyint(ldexp(a, 24)));
+}
+
+uint16_t fl_f64_to_bf16(double v)
+{
+    if (isnan(v)) return 0x7fc0;
+    double r = v;
+    if (v != 0.0 && isfinite(v)) {
+        int e;
+        const double m = frexp(v, &e);
+        r = ldexp(nearbyint(ldexp(m, 8)), e - 8); /* 8 significant bits, ties to even */
+    }
+    const float f = (float)r; /* exact within the float range */
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return (uint16_t)(u >> 16);
+}
+
+void fl_pack_coefs16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                     const float *delta, const float *phi, const float *theta, double dt, int bf16,
+                     uint16_t *const *out)
+{
+    const double dt2 = dt * dt;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const double st = sin((double)theta[i]), ct = cos((double)theta[i]);
+        const double s2t = sin(2.0 * theta[i]);
+        const double sp = sin((double)phi[i]), cp = cos((double)phi[i]);
+        const double s2p = sin(2.0 * phi[i]);
+        const double vp2 = (double)vpz[i] * vpz[i];
+        const double c[10] = {st * st * cp * cp, st * st * sp * sp, ct * ct, st * st * s2p,
+                              s2t * sp, s2t * cp, vp2 * (1.0 + 2.0 * epsilon[i]) * dt2, vp2 * dt2,
+                              (double)vsv[i] * vsv[i] * dt2, vp2 * (1.0 + 2.0 * delta[i]) * dt2};
+        for (int k = 0; k < 10; k++) out[k][i] = bf16 ? fl_f64_to_bf16(c[k]) : fl_f64_to_f16(c[k]);
+    }
+}
+
+void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                         const float *delta, const float *phi, const float *theta, double dt,
+                         uint16_t *const *out)
+{
+    const double pi = 3.14159265358979323846;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        out[0][i] = fl_f64_to_f16((double)vpz[i] * dt);
+        out[1][i] = fl_f64_to_f16((double)vsv[i] * dt);
+        out[2][i] = fl_f64_to_f16((double)epsilon[i]);
+        out[3][i] = fl_f64_to_f16((double)delta[i]);
+        out[4][i] = fl_f64_to_f16((double)theta[i]);
+        out[5][i] = fl_f64_to_f16((double)phi[i] - pi);
+    }
+}

@@ -56,6 +56,20 @@ void fl_precompute_f64(int64_t n, const float *vpz, const float *vsv, const floa
                        const float *delta, const float *phi, const float *theta,
                        double *const *coef);
 
+/* 16-bit coefficient storage (no reference counterpart).  The 10 coefficients of
+ * fl_precompute_f64 with v2px, v2pz, v2sz, v2pn multiplied by dt^2, each rounded once
+ * (round-to-nearest-even) from FP64 to BF16 (bf16 != 0) or FP16; out[k] hold n values. */
+void fl_pack_coefs16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                     const float *delta, const float *phi, const float *theta, double dt, int bf16,
+                     uint16_t *const *out);
+/* Inputs of the in-kernel recompute, FP16: vpz*dt, vsv*dt, eps, delta, theta, phi-pi. */
+void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                         const float *delta, const float *phi, const float *theta, double dt,
+                         uint16_t *const *out);
+/* Single conversions (round to nearest even), exported for tests. */
+uint16_t fl_f64_to_f16(double v);
+uint16_t fl_f64_to_bf16(double v);
+
 /* Number of time steps ceil(tmax/dt) in float, main.c:101. */
 int fl_num_steps(float tmax, float dt);
 

@@ -158,3 +158,50 @@ def test_rsf_writer_layout(fl, tmp_path):
     assert 'data_format="native_float"' in head and "n1=3" in head and "n2=4" in head and "n3=3" in head
     data = np.fromfile(name + ".rsf@", dtype=np.float32).reshape(3, 4, 3)
     assert np.array_equal(data, a[:, :, 1:4])
+
+
+def test_f64_to_f16_round_to_nearest_even(fl):
+    """Against numpy's correctly rounded float64 -> float16 conversion, including ties,
+    subnormals, overflow and signed zero."""
+    rng = np.random.default_rng(1)
+    vals = np.concatenate([rng.standard_normal(4000) * 10.0 ** rng.integers(-9, 6, 4000),
+                           [0.0, -0.0, 65504.0, 65519.99, 65520.0, 1e6, 2.0 ** -24, 2.0 ** -25,
+                            3 * 2.0 ** -26, 1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11, 6.1e-5, -7e-6]])
+    L = fl.lib()
+    mine = np.array([L.fl_f64_to_f16(float(v)) for v in vals], np.uint16)
+    assert np.array_equal(mine, vals.astype(np.float16).view(np.uint16))
+
+
+def test_f64_to_bf16_is_nearest_with_ties_to_even(fl):
+    rng = np.random.default_rng(2)
+    vals = np.concatenate([rng.standard_normal(4000) * 10.0 ** rng.integers(-20, 20, 4000),
+                           [1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -(1.0 + 2.0 ** -8), 0.0, 3.0e38]])
+    L = fl.lib()
+    for v in vals:
+        b = L.fl_f64_to_bf16(float(v))
+        got = np.array([b << 16], np.uint32).view(np.float32)[0]
+        # the two bf16 values bracketing v: truncation toward zero and the next one out
+        t = np.array([v], np.float32).view(np.uint32)[0] & 0xFFFF0000
+        lo = np.array([t], np.uint32).view(np.float32)[0]
+        hi = np.array([t + 0x10000], np.uint32).view(np.float32)[0]
+        cands = [(abs(float(c) - v), (int(t if c == lo else t + 0x10000) >> 16) & 1, float(c))
+                 for c in (lo, hi)]
+        best = min(cands)
+        assert float(got) == best[2] or (cands[0][0] == cands[1][0] and ((b & 1) == 0)), v
+
+
+def test_pack_coefs16_matches_fp64_precompute(fl):
+    m = fl.model_gradient((16, 14, 12), 4, seed=9)
+    n = m[0].size
+    dt = 7.5e-4
+    c64 = fl.precompute(m, "f64")
+    for bf16 in (0, 1):
+        out = [np.empty(m[0].shape, np.uint16) for _ in range(10)]
+        fl.lib().fl_pack_coefs16(n, *m, dt, bf16, (C.c_void_p * 10)(*[o.ctypes.data for o in out]))
+        for k in range(10):
+            ref = c64[k] * (dt * dt if k >= 6 else 1.0)
+            if bf16:
+                got = (out[k].astype(np.uint32) << 16).view(np.float32)
+                assert np.allclose(got, ref, rtol=2 ** -8, atol=1e-30)
+            else:
+                assert np.array_equal(out[k], ref.astype(np.float16).view(np.uint16))

@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "fl_zmarch_sf.h"
+#include "fl_step.h"
 #include "fletcher/driver.h"
 #include "fletcher/fl_hip.h"
 #include "fletcher/fl_model.h"
@@ -84,11 +84,15 @@ static Tiles g_tiles;
 static bool g_timing = false;
 static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_ev_used, g_ev_free;
 static hipEvent_t g_reg0 = nullptr, g_reg1 = nullptr;
-// step kernel family: 1 split-field (one field per half wave, 32-column tiles),
-// 0 one lane per cell (64-column tiles); FL_KERNEL=cell|sf selects it
-static int g_sf = 1;
+// tile width of the step kernel (columns per wave and field)
+static int tile_x() { return 32; }
 
-static int tile_x() { return g_sf ? 32 : 64; }
+// Deepest z chunk whose pc/qc span (zt + 8 planes) fits a buffer range (< 2^31 bytes).
+static int max_zt(int sx, int sy)
+{
+    const long long pb = (long long)sx * sy * 4;
+    return (int)std::max<long long>(0, (0x7fffffffll / pb) - 8);
+}
 
 static int ensure_stream()
 {
@@ -99,8 +103,6 @@ static int ensure_stream()
     HIPCHK(hipSetDevice(0));
     // default flags: the stream also orders after legacy null-stream work (hipMemcpy etc.)
     HIPCHK(hipStreamCreate(&g_stream));
-    const char *k = getenv("FL_KERNEL");
-    if (k && strcmp(k, "cell") == 0) g_sf = 0;
     return FL_OK;
 }
 
@@ -127,27 +129,32 @@ extern "C" int fl_bytes_per_cell(void)
     return 24 + c;
 }
 
-// Host precompute of one z slab [z0,z1) into device coefficient storage.
+// Host precompute into the device coefficient records: per cell, its coef_count(cm)
+// values in precomp.h order (FP32) or the 16-bit forms, consecutive (record of
+// coef_count * coef_elem bytes); z slabs of at most 64 MiB per array on the host.
 static int upload_coefs(const float *vpz, const float *vsv, const float *eps, const float *del,
                         const float *phi, const float *theta, float dt, int cm, void *dst,
                         long long n, long long sxy, int sz)
 {
+    (void)n;
     const int slab = std::max(1, (int)std::min<long long>(sz, (64ll << 20) / (sxy * 4)));
-    std::vector<float> f32;
-    std::vector<uint16_t> h16;
+    const int nk = coef_count(cm);
+    std::vector<float> f32, r32;
+    std::vector<uint16_t> h16, r16;
     for (int z0 = 0; z0 < sz; z0 += slab) {
         const int z1 = std::min(sz, z0 + slab);
         const long long off = (long long)z0 * sxy, m = (long long)(z1 - z0) * sxy;
         if (cm == CM_F32) {
             f32.resize(10 * m);
+            r32.resize(10 * m);
             float *c[10];
             for (int k = 0; k < 10; k++) c[k] = f32.data() + k * m;
             fl_precompute_f32(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, c);
-            for (int k = 0; k < 10; k++)
-                HIPCHK(hipMemcpy((float *)dst + k * n + off, c[k], m * 4, hipMemcpyHostToDevice));
+            fl_interleave32(m, 10, c, r32.data());
+            HIPCHK(hipMemcpy((float *)dst + off * 10, r32.data(), m * 40, hipMemcpyHostToDevice));
         } else {
-            const int nk = coef_count(cm);
             h16.resize((size_t)nk * m);
+            r16.resize((size_t)nk * m);
             uint16_t *c[10];
             for (int k = 0; k < nk; k++) c[k] = h16.data() + k * m;
             if (cm == CM_F16R)
@@ -155,8 +162,8 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
             else
                 fl_pack_coefs16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
                                 cm == CM_BF16, c);
-            for (int k = 0; k < nk; k++)
-                HIPCHK(hipMemcpy((uint16_t *)dst + k * n + off, c[k], m * 2, hipMemcpyHostToDevice));
+            fl_interleave16(m, nk, c, r16.data());
+            HIPCHK(hipMemcpy((uint16_t *)dst + off * nk, r16.data(), m * nk * 2, hipMemcpyHostToDevice));
         }
     }
     return FL_OK;
@@ -172,7 +179,7 @@ static int default_zt(int sx, int sy, int sz, int bord, int ty)
     int nzc = (int)std::max<long long>(1, (target + per_plane - 1) / per_plane);
     int zt = (nzu + nzc - 1) / nzc;
     zt = std::max(zt, 16);
-    return std::min(zt, nzu);
+    return std::min(std::min(zt, nzu), max_zt(sx, sy));
 }
 
 static void fill_geom(StepArgs &a, int sx, int sy, int sz, int bord, int ty, int zt)
@@ -189,13 +196,10 @@ template <int TY, int CM, bool TP>
 static void launch_zm(const StepArgs &a, hipStream_t s)
 {
     const int g = (a.nblk + 7) / 8 * 8;
-    if (g_sf)
-        hipLaunchKernelGGL((zmarch_sf<TY, CM, TP>), dim3(g), dim3(64, TY), 0, s, a);
-    else
-        hipLaunchKernelGGL((zmarch<TY, CM, TP>), dim3(g), dim3(64, TY), 0, s, a);
+    hipLaunchKernelGGL((tti_step<TY, CM, TP>), dim3(g), dim3(64, TY), 0, s, a);
 }
 
-static int launch_zmarch(const StepArgs &a, int ty, int cm, bool tp, hipStream_t s)
+static int launch_steps(const StepArgs &a, int ty, int cm, bool tp, hipStream_t s)
 {
 #define FL_ZM(TY)                                                          \
     if (ty == TY) {                                                        \
@@ -220,7 +224,6 @@ static StepArgs base_args()
     memset(&a, 0, sizeof(a));
     a.pp = D.f[0]; a.pc = D.f[1]; a.qp = D.f[2]; a.qc = D.f[3];
     a.coef = D.coef;
-    a.coef_stride = D.n;
     a.dvol = D.dvol;
     a.dxxinv = 1.0f / (D.dx * D.dx);
     a.dyyinv = 1.0f / (D.dy * D.dy);
@@ -229,7 +232,7 @@ static StepArgs base_args()
     a.dyinv = 1.0f / D.dy;
     a.dzinv = 1.0f / D.dz;
     a.dt = D.dt;
-    const int zt = g_tiles.zt > 0 ? std::min(g_tiles.zt, D.sz - 2 * D.bord)
+    const int zt = g_tiles.zt > 0 ? std::min(std::min(g_tiles.zt, D.sz - 2 * D.bord), max_zt(D.sx, D.sy))
                                   : default_zt(D.sx, D.sy, D.sz, D.bord, g_tiles.ty);
     fill_geom(a, D.sx, D.sy, D.sz, D.bord, g_tiles.ty, zt);
     a.src_idx = -1;
@@ -260,7 +263,7 @@ static int launch_step(StepArgs a, int it_host, hipStream_t s)
 {
     a.it_host = it_host;
     if (g_variant == FL_VARIANT_FUSED) {
-        return launch_zmarch(a, g_tiles.ty, D.cm, false, s);
+        return launch_steps(a, g_tiles.ty, D.cm, false, s);
     }
     if (D.cm != CM_F32)
         return fail(FL_ERR_ARG, "variant %d needs FP32 coefficients", g_variant);
@@ -271,13 +274,13 @@ static int launch_step(StepArgs a, int it_host, hipStream_t s)
         dim3 g1((D.sx + 63) / 64, (D.sy + 3) / 4, D.sz);
         hipLaunchKernelGGL(der1_pass, g1, dim3(64, 4), 0, s, a.pc, a.qc, D.dvol, D.sx, D.sy, D.sz,
                            D.bord, a.dxinv, a.dyinv);
-        return launch_zmarch(a, g_tiles.ty, CM_F32, true, s);
+        return launch_steps(a, g_tiles.ty, CM_F32, true, s);
     }
     if (g_variant == FL_VARIANT_DIRECT) {
         DirectArgs d;
         d.pp = a.pp; d.pc = a.pc; d.qp = a.qp; d.qc = a.qc;
         d.coef = (const float *)D.coef;
-        d.coef_stride = D.n;
+
         d.dxxinv = a.dxxinv; d.dyyinv = a.dyyinv; d.dzzinv = a.dzzinv;
         d.dxyinv = 1.0f / (D.dx * D.dy);
         d.dxzinv = 1.0f / (D.dx * D.dz);
@@ -341,6 +344,8 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
         return fail(FL_ERR_ARG, "spacings and dt must be positive");
     if (!vpz || !vsv || !eps || !del || !phi || !theta)
         return fail(FL_ERR_ARG, "model arrays must not be NULL");
+    if (max_zt(sx, sy) < 1 || (long long)sx * sy * 40 >= 0x7fffffffll)
+        return fail(FL_ERR_ARG, "xy plane %dx%d too large for 32-bit plane offsets", sx, sy);
     int rc = ensure_stream();
     if (rc) return rc;
     free_device();
@@ -603,7 +608,7 @@ extern "C" int fl_run_steps(int it0, int nsteps, int iSource, const float *src, 
         HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
         int rc = FL_OK;
         for (int k = 0; k < K && rc == FL_OK; k++) {
-            rc = launch_zmarch(step_args(k), g_tiles.ty, D.cm, false, g_stream);
+            rc = launch_steps(step_args(k), g_tiles.ty, D.cm, false, g_stream);
             swap_levels();
         }
         hipError_t ec = hipStreamEndCapture(g_stream, &graph);
@@ -618,7 +623,7 @@ extern "C" int fl_run_steps(int it0, int nsteps, int iSource, const float *src, 
         (void)hipGraphDestroy(graph);
     }
     for (int k = done; k < nsteps; k++) {
-        int rc = launch_zmarch(step_args(k), g_tiles.ty, D.cm, false, g_stream);
+        int rc = launch_steps(step_args(k), g_tiles.ty, D.cm, false, g_stream);
         if (rc) return rc;
         swap_levels();
     }
@@ -771,7 +776,7 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
     double best = 1e30;
     Tiles bt = keep;
     for (int cty : {4, 8, 16}) {
-        for (int czt : zt_candidates(D.sz - 2 * D.bord)) {
+        for (int czt : zt_candidates(std::min(D.sz - 2 * D.bord, max_zt(D.sx, D.sy)))) {
             g_tiles.ty = cty;
             g_tiles.zt = czt;
             StepArgs a = base_args();

@@ -5,19 +5,12 @@
 //   pp <- 2 pc - pp + dt^2 rhsp(pc, qc),   qp <- 2 qc - qp + dt^2 rhsq(pc, qc)
 // over iz,iy,ix in [bord, s-bord) (original/OpenMP/openmp_propagate.c:27-29).
 //
-// Kernels:
-//   zmarch<TY, CM, TWOPASS>  the drop-in: a 64 x TY xy tile per workgroup marches
-//       z over a chunk.  The incoming plane f = z+4 of pc/qc is staged in LDS with
-//       a 4-cell halo; first derivatives Dx, Dy of that plane are formed on chip;
-//       9-deep register queues hold pc, qc, Dx, Dy along z, and a 4-deep queue
-//       delays the in-plane terms (Dxx, Dyy, Dxy) until plane z is complete.
-//       Mixed derivatives are Der1 of Der1 (der1der1/sample.h:57-62, 83-88);
-//       TWOPASS reads Dx, Dy from the volumes written by der1_pass instead.
-//       CM selects the coefficient storage (FP32 / BF16 / FP16 / FP16 recompute).
+// Kernels here (the z-marching step kernel is in fl_step.h):
 //   direct_dercross  the reference's direct 64-point DerCross form
 //       (original/derivatives.h:59-68), one thread per (x,y) column.
 //   der1_pass        pass 1 of the two-pass form (der1der1/OpenMP/openmp_propagate.c:24-76).
 //   insert_source    p[i] += s, q[i] += s (original/OpenMP/openmp_insertsource.c:7-12).
+//   gather_plane     receiver plane copy.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -58,8 +51,7 @@ struct StepArgs {
     const float *pc;
     float *qp;
     const float *qc;
-    const void *coef;     // coefficient volumes, consecutive, coef_stride elements apart
-    long long coef_stride;
+    const void *coef;     // coefficient records: per cell its values, consecutive
     const float *dvol;    // TWOPASS: pDx, pDy, qDx, qDy volumes, n elements apart
     float dxxinv, dyyinv, dzzinv, dxinv, dyinv, dzinv, dt;
     int sx, sy, sz, bord;
@@ -160,46 +152,6 @@ struct Coef {
     float c[10];
 };
 
-template <int CM>
-__device__ __forceinline__ void load_coef(const void *base, long long stride, long long i, Coef &k)
-{
-    if constexpr (CM == CM_F32) {
-        const float *b = (const float *)base;
-#pragma unroll
-        for (int j = 0; j < 10; j++) k.c[j] = __builtin_nontemporal_load(b + j * stride + i);
-    } else if constexpr (CM == CM_BF16 || CM == CM_F16) {
-        const uint16_t *b = (const uint16_t *)base;
-#pragma unroll
-        for (int j = 0; j < 10; j++) {
-            const uint16_t h = __builtin_nontemporal_load(b + j * stride + i);
-            k.c[j] = (CM == CM_BF16) ? bf16_to_f32(h) : f16_to_f32(h);
-        }
-    } else {  // CM_F16R: recompute from vpz*dt, vsv*dt, eps, delta, theta, phi-pi
-        const uint16_t *b = (const uint16_t *)base;
-        float r[6];
-#pragma unroll
-        for (int j = 0; j < 6; j++) r[j] = f16_to_f32(__builtin_nontemporal_load(b + j * stride + i));
-        const float w2 = r[0] * r[0];
-        float st, ct, spc, cpc;
-        __sincosf(r[4], &st, &ct);
-        __sincosf(r[5], &spc, &cpc);
-        const float sp = -spc, cp = -cpc;  // phi = phic + pi
-        const float st2 = st * st;
-        const float s2t = 2.0f * st * ct;
-        const float s2p = 2.0f * sp * cp;
-        k.c[0] = st2 * cp * cp;
-        k.c[1] = st2 * sp * sp;
-        k.c[2] = ct * ct;
-        k.c[3] = st2 * s2p;
-        k.c[4] = s2t * sp;
-        k.c[5] = s2t * cp;
-        k.c[6] = w2 * (1.0f + 2.0f * r[2]);
-        k.c[7] = w2;
-        k.c[8] = r[1] * r[1];
-        k.c[9] = w2 * (1.0f + 2.0f * r[3]);
-    }
-}
-
 // Coefficient combination + leapfrog, association order of original/sample.h:46-86.
 template <int CM>
 __device__ __forceinline__ void combine(const Coef &k, float dt, float pxx, float pyy, float pzz,
@@ -278,8 +230,7 @@ struct DirectArgs {
     const float *pc;
     float *qp;
     const float *qc;
-    const float *coef;
-    long long coef_stride;
+    const float *coef;    // FP32 records of 10 per cell
     float dxxinv, dyyinv, dzzinv, dxyinv, dxzinv, dyzinv, dt;
     int sx, sy, sz, bord, zt;
     float *trace;
@@ -311,7 +262,7 @@ __global__ __launch_bounds__(256) void direct_dercross(const DirectArgs a)
         const float qxz = dercross_g(a.qc, i, sX, sZ, a.dxzinv);
         Coef k;
 #pragma unroll
-        for (int j = 0; j < 10; j++) k.c[j] = a.coef[j * a.coef_stride + i];
+        for (int j = 0; j < 10; j++) k.c[j] = a.coef[i * 10 + j];
         float ppv = a.pp[i], qpv = a.qp[i];
         combine<CM_F32>(k, a.dt, pxx, pyy, pzz, pxy, pyz, pxz, qxx, qyy, qzz, qxy, qyz, qxz,
                         a.pc[i], a.qc[i], ppv, qpv);

@@ -1,0 +1,513 @@
+// The TTI step kernel for gfx950: split-field z-marching with LDS plane rings.
+//
+// Work decomposition.  A workgroup of TY waves owns a 32 x TY tile of the xy plane and a
+// chunk [zlo, zhi) of z; it streams the input planes f = zlo-4 .. zhi+3 of pc, qc through
+// LDS once.  A wave covers 32 columns of one tile row for BOTH fields: lanes 0-31 carry p,
+// lanes 32-63 carry q of the same 32 cells, so each lane keeps only its own field's state.
+//
+// On chip, per lane (own field, own column):
+//   registers  9-deep z queues of c = pc|qc, Dx and Dy over planes f-8 .. f (Dzz, Dxz, Dyz
+//              of the output plane z = f-4: der1der1lm's 9-plane ring,
+//              der1der1lm/zTileSize.h:4-7 and sample.h:147-160, held in registers).  The
+//              ring is addressed statically: the steady loop is unrolled 9 times, one copy
+//              per ring phase, so planes advance without register moves.
+//   LDS        a ring of 6 staged planes of pc, qc (tile + 4-cell halo): plane f for Dx, Dy
+//              and the queues, plane z = f-4 for Dxx, Dyy, plane f+1 being written;
+//              a ring of 5 planes of Dx over the tile rows plus the 4-row y halo, from which
+//              Dxy(z) = Der1_y(Dx) is read (der1der1/sample.h:57-62, 83-88).
+// With the rings every plane needs a single barrier: nothing written in a step is read
+// in it, and nothing read in it is overwritten before the next barrier.
+//
+// Memory.  Every stream goes through a buffer resource over the chunk's span with a
+// wave-uniform plane offset (soffset): one SGPR add per plane, no 64-bit address math.
+// Lanes outside the grid carry an offset past the range, so their loads return 0 and
+// their stores are dropped by the hardware (no masks, no clamps).
+//
+// The combination and leapfrog follow original/sample.h:46-86 in its association order;
+// p and q exchange h1q / h2p with one cross-half lane swap (v_permlane32_swap).
+//
+// TWOPASS takes Dx, Dy from the four first-derivative volumes of der1_pass
+// (der1der1/OpenMP/openmp_propagate.c:24-76) instead of forming them on chip.
+#pragma once
+#include <type_traits>
+#include <utility>
+
+#include "fl_kernels.h"
+
+namespace fl {
+
+// Global load through a wave-uniform base pointer plus a 32-bit lane byte offset.
+__device__ __forceinline__ float ld_f32(const void *base, uint32_t off)
+{
+    return *(const float *)((const char *)base + off);
+}
+
+// Lane byte offset beyond every buffer range used here (spans are < 2^31 bytes).
+constexpr uint32_t FL_OOB = 0x80000000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t span_rsrc(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+__device__ __forceinline__ uint32_t bload32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+
+// Coefficient record of one cell as loaded: FP32 10 values (40 B); BF16 / FP16 10 values
+// (20 B); FP16 recompute inputs 6 values (12 B).  Kept raw until used, so the loads stay
+// in flight across barriers.
+template <int CM>
+struct CoefRec {
+    static constexpr int NW = (CM == CM_F32) ? 10 : (CM == CM_F16R ? 3 : 5);
+    uint32_t w[NW];
+};
+
+template <int CM>
+__device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, uint32_t voff, CoefRec<CM> &c)
+{
+    if constexpr (CM == CM_F32) {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16u, 0, 0);
+        const auto d = __builtin_amdgcn_raw_buffer_load_b64(r, voff + 32u, 0, 0);
+        c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
+        c.w[4] = b[0]; c.w[5] = b[1]; c.w[6] = b[2]; c.w[7] = b[3];
+        c.w[8] = d[0]; c.w[9] = d[1];
+    } else if constexpr (CM == CM_F16R) {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b96(r, voff, 0, 0);
+        c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2];
+    } else {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+        c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
+        c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, voff + 16u, 0, 0);
+    }
+}
+
+template <int CM>
+__device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, Coef &k)
+{
+    if constexpr (CM == CM_F32) {
+#pragma unroll
+        for (int j = 0; j < 10; j++) k.c[j] = __builtin_bit_cast(float, r.w[j]);
+    } else if constexpr (CM == CM_BF16) {
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            k.c[2 * j] = __builtin_bit_cast(float, r.w[j] << 16);
+            k.c[2 * j + 1] = __builtin_bit_cast(float, r.w[j] & 0xffff0000u);
+        }
+    } else if constexpr (CM == CM_F16) {
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            k.c[2 * j] = f16_to_f32((uint16_t)r.w[j]);
+            k.c[2 * j + 1] = f16_to_f32((uint16_t)(r.w[j] >> 16));
+        }
+    } else {  // CM_F16R: rebuild from vpz*dt, vsv*dt, eps, delta, theta, phi-pi
+        float v[6];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            v[2 * j] = f16_to_f32((uint16_t)r.w[j]);
+            v[2 * j + 1] = f16_to_f32((uint16_t)(r.w[j] >> 16));
+        }
+        const float w2 = v[0] * v[0];
+        float st, ct, spc, cpc;
+        __sincosf(v[4], &st, &ct);
+        __sincosf(v[5], &spc, &cpc);
+        const float sp = -spc, cp = -cpc;  // phi = phic + pi
+        const float st2 = st * st;
+        const float s2t = 2.0f * st * ct;
+        const float s2p = 2.0f * sp * cp;
+        k.c[0] = st2 * cp * cp;
+        k.c[1] = st2 * sp * sp;
+        k.c[2] = ct * ct;
+        k.c[3] = st2 * s2p;
+        k.c[4] = s2t * sp;
+        k.c[5] = s2t * cp;
+        k.c[6] = w2 * (1.0f + 2.0f * v[2]);
+        k.c[7] = w2;
+        k.c[8] = v[1] * v[1];
+        k.c[9] = w2 * (1.0f + 2.0f * v[3]);
+    }
+}
+
+template <int TY>
+struct StepLds {
+    static constexpr int HX = 40, HY = TY + 8, HT = HX * HY, XT = 32 * HY;
+    float t[6][2 * HT];  // staged pc | qc planes, tile + halo, p then q
+    float x[5][2 * XT];  // Dx of pc | qc, tile columns x (tile rows + y halo)
+};
+
+template <int V>
+using ic = std::integral_constant<int, V>;
+
+template <class F, int... P>
+__device__ __forceinline__ void unroll_phases(F &&fn, std::integer_sequence<int, P...>)
+{
+    (fn(ic<P>{}), ...);
+}
+
+// One tile of one z chunk.  Output planes z in [zlo, zhi) of pp/qp are written once.
+template <int TY, int CM, bool TWOPASS>
+__device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl, int tix, int tiy, int zc,
+                                          StepLds<TY> &s)
+{
+    constexpr int TX = 32, HX = 40, HY = TY + 8, HT = HX * HY, XT = TX * HY, NT = 64 * TY;
+    constexpr int HTP = (HT + 63) / 64 * 64;  // field stride of the staging index space
+    constexpr int NLD = (2 * HTP + NT - 1) / NT;
+    constexpr int REC = (CM == CM_F32) ? 40 : (CM == CM_F16R ? 12 : 20);  // record bytes
+    constexpr int NH = (8 + TY - 1) / TY;     // y-halo rows of Dx per wave (at most)
+    const int x0 = tix * TX, y0 = a.bord + tiy * TY;
+    const int zlo = a.bord + zc * a.zt;
+    const int zhi = min(zlo + a.zt, a.sz - a.bord);
+    const int nz = zhi - zlo, f0 = zlo - 4;
+    const int lane = threadIdx.x, ly = threadIdx.y;
+    const int wv = __builtin_amdgcn_readfirstlane(ly);
+    const int fld = lane >> 5, cx = lane & 31;
+    const int x = x0 + cx, y = y0 + ly;
+    const bool active = (x >= a.bord) && (x < a.sx - a.bord) && (y < a.sy - a.bord);
+    const uint32_t sxy = (uint32_t)a.sx * (uint32_t)a.sy;
+    const uint32_t pb4 = sxy * 4u, pbr = sxy * (uint32_t)REC;
+    const uint32_t col = (uint32_t)(y * a.sx + x);
+    // clamped into the grid (TWOPASS global loads of lanes outside the update range)
+    const uint32_t colr = (uint32_t)(min(y, a.sy - 1) * a.sx + min(x, a.sx - 1));
+    const uint32_t vo4 = active ? col * 4u : FL_OOB;
+    const uint32_t vor = active ? col * (uint32_t)REC : FL_OOB;
+    const uint32_t vop = fld ? FL_OOB : vo4, voq = fld ? vo4 : FL_OOB;  // own field only
+
+    const long long zin = (long long)f0 * sxy, zout = (long long)zlo * sxy;
+    const __amdgpu_buffer_rsrc_t r_pc = span_rsrc(a.pc + zin, (uint32_t)(nz + 8) * pb4);
+    const __amdgpu_buffer_rsrc_t r_qc = span_rsrc(a.qc + zin, (uint32_t)(nz + 8) * pb4);
+    const __amdgpu_buffer_rsrc_t r_pp = span_rsrc(a.pp + zout, (uint32_t)nz * pb4);
+    const __amdgpu_buffer_rsrc_t r_qp = span_rsrc(a.qp + zout, (uint32_t)nz * pb4);
+    const char *cpl = (const char *)a.coef + zout * REC;  // records of the next output plane
+
+    // source cell of this step (added to both fields of the output) and receiver plane
+    int src_z = -1;
+    uint32_t src_col = 0;
+    if (ctl.do_src && a.src_idx >= 0) {
+        src_z = (int)(a.src_idx / (long long)sxy);
+        src_col = (uint32_t)(a.src_idx - (long long)src_z * sxy);
+    }
+    const int rec_z = ctl.trace ? a.rec_iz : -1;
+
+    // staging: element e' = wave*64 + k*NT + lane of [0, 2*HTP); p for e' < HTP, q above;
+    // the field of a (wave, k) pair is uniform
+    uint32_t hoff[NLD];
+    int hl[NLD];
+    bool hq[NLD], hk[NLD];
+#pragma unroll
+    for (int k = 0; k < NLD; k++) {
+        const int ew = wv * 64 + k * NT;
+        hk[k] = ew < 2 * HTP;
+        hq[k] = ew >= HTP;
+        const int ee = ew + lane - (hq[k] ? HTP : 0);
+        const bool inb = hk[k] && (HT == HTP || ee < HT);
+        const int r = ee / HX, c = ee - r * HX;
+        const int gy = y0 - 4 + r, gx = x0 - 4 + c;
+        const bool ok = inb && gx >= 0 && gx < a.sx && gy < a.sy;
+        hoff[k] = ok ? (uint32_t)(gy * a.sx + gx) * 4u : FL_OOB;
+        hl[k] = (hq[k] ? HT : 0) + ee;
+    }
+    float pre[NLD];
+    auto fetch = [&](int i) {  // raw loads of input plane f0 + i
+        const uint32_t so = (uint32_t)i * pb4;
+#pragma unroll
+        for (int k = 0; k < NLD; k++)
+            if (hk[k]) pre[k] = __builtin_bit_cast(float, bload32(hq[k] ? r_qc : r_pc, hoff[k], so));
+    };
+    auto stage = [&](float *T) {
+#pragma unroll
+        for (int k = 0; k < NLD; k++)
+            if (hk[k] && (HT == HTP || hl[k] < (hq[k] ? 2 * HT : HT))) T[hl[k]] = pre[k];
+    };
+
+    // lane offsets in the LDS planes
+    // (window corners, so that every stencil read is a non-negative immediate offset)
+    const int ow = fld * HT + ly * HX + cx;  // 9 x 9 window of the own cell in a staged plane
+    const int oxw = fld * XT + ly * TX + cx; // 9-row column of the own cell in a Dx plane
+    constexpr int CW = 4 * HX + 4, CX = 4 * TX;  // the own cell within them
+    int ohr[NH], ohx[NH];                              // y-halo rows of Dx done by this wave
+    bool hrow[NH];
+#pragma unroll
+    for (int m = 0; m < NH; m++) {
+        const int j = wv + m * TY;  // halo row index 0..7 (rows 0-3 and TY+4..TY+7)
+        hrow[m] = j < 8;
+        const int r = j < 4 ? j : TY + j;
+        ohr[m] = fld * HT + r * HX + cx;
+        ohx[m] = fld * XT + r * TX + cx;
+    }
+
+    // TWOPASS: first-derivative volumes of the own field
+    const long long nvol = (long long)sxy * a.sz;
+    const float *dvx = TWOPASS ? a.dvol + (fld ? 2 * nvol : 0) : nullptr;
+    const float *dvy = TWOPASS ? a.dvol + (fld ? 3 * nvol : nvol) : nullptr;
+
+    float qc_[9], qx_[9], qy_[9];  // own field c, Dx, Dy of planes f-8..f (ring)
+#pragma unroll
+    for (int k = 0; k < 9; k++) qc_[k] = qx_[k] = qy_[k] = 0.0f;
+
+    // prologue: plane f0 into ring slot 0, loads of plane f0+1 in flight
+    fetch(0);
+    stage(s.t[0]);
+    fetch(1);
+    __syncthreads();
+
+    int r6 = 0, r5 = 0;  // LDS ring slots of plane f0 + i: i % 6, i % 5
+    // one input plane: PH ring phase, XW 0/1/2 = Dx halo planes never/always/if i < nz+4,
+    // OUT output plane z = f-4, GEN generic copy (phase 0, queues shifted afterwards)
+    auto step = [&](auto PHc, auto XWc, auto OUTc, auto GENc, const int i) {
+        constexpr int PH = decltype(PHc)::value;
+        constexpr int XW = decltype(XWc)::value;
+        constexpr bool OUT = decltype(OUTc)::value != 0, GEN = decltype(GENc)::value != 0;
+        const int f = f0 + i;
+        const int z = f - 4;
+        const int r6n = r6 == 5 ? 0 : r6 + 1, r5n = r5 == 4 ? 0 : r5 + 1;
+        float *const Tf = s.t[r6];
+        float *const Tz = s.t[r6 >= 4 ? r6 - 4 : r6 + 2];
+        float *const Xf = s.x[r5];
+        float *const Xz = s.x[r5n];
+        const uint32_t so_o = (uint32_t)(i - 8) * pb4;
+
+        // streams of the output plane, kept raw until used
+        CoefRec<CM> kr;
+        float vpp = 0.0f, vqp = 0.0f;
+        if constexpr (OUT) {
+            load_rec<CM>(span_rsrc(cpl, pbr), vor, kr);
+            cpl += pbr;
+            vpp = __builtin_bit_cast(float, bload32(r_pp, vop, so_o));
+            vqp = __builtin_bit_cast(float, bload32(r_qp, voq, so_o));
+        }
+        const bool xw = (XW == 1) || (XW == 2 && i < nz + 4);
+        float nx, ny, gxh[NH];
+        if constexpr (TWOPASS) {
+            const long long fo = (long long)f * sxy;
+            nx = ld_f32(dvx + fo, colr * 4u);
+            ny = ld_f32(dvy + fo, colr * 4u);
+            if (xw) {
+#pragma unroll
+                for (int m = 0; m < NH; m++)
+                    if (hrow[m]) {
+                        const int r = (wv + m * TY) < 4 ? (wv + m * TY) : TY + wv + m * TY;
+                        const int gy = y0 - 4 + r;
+                        gxh[m] = (x < a.sx && gy < a.sy) ? ld_f32(dvx + fo, (uint32_t)(gy * a.sx + x) * 4u) : 0.0f;
+                    }
+            }
+        }
+        const float *cp = Tf + ow + CW;
+        const float nc = cp[0];
+        if constexpr (!TWOPASS) {
+            nx = der1_lds(cp, 1, a.dxinv);
+            ny = der1_lds(cp, HX, a.dyinv);
+        }
+        if (xw) {
+            Xf[oxw + CX] = nx;
+#pragma unroll
+            for (int m = 0; m < NH; m++)
+                if (hrow[m]) {
+                    if constexpr (TWOPASS) Xf[ohx[m]] = gxh[m];
+                    else Xf[ohx[m]] = der1_lds(Tf + ohr[m] + 4, 1, a.dxinv);
+                }
+        }
+        constexpr int ins = GEN ? 8 : (PH + 8) % 9;
+        qc_[ins] = nc;
+        qx_[ins] = nx;
+        qy_[ins] = ny;
+
+        float dxx = 0.f, dyy = 0.f, dxy = 0.f;
+        if constexpr (OUT) {
+            const float *zp = Tz + ow + CW;
+            dxx = der2_lds(zp, 1, a.dxxinv);
+            dyy = der2_lds(zp, HX, a.dyyinv);
+            dxy = der1_lds(Xz + oxw + CX, TX, a.dyinv);
+        }
+        // next plane into the ring, the one after it in flight
+        if (i + 1 <= nz + 7) stage(s.t[r6n]);
+        if (i + 2 <= nz + 7) fetch(i + 2);
+
+        if constexpr (OUT) {
+            constexpr int b = GEN ? 0 : PH;
+            float qz[9], qzx[9], qzy[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                qz[k] = qc_[(b + k) % 9];
+                qzx[k] = qx_[(b + k) % 9];
+                qzy[k] = qy_[(b + k) % 9];
+            }
+            const float dzz = der2_q(qz, a.dzzinv);
+            const float dxz = der1_q(qzx, a.dzinv);
+            const float dyz = der1_q(qzy, a.dzinv);
+            Coef kc;
+            decode_rec<CM>(kr, kc);
+            // H1, H2 of the own field, association order of original/sample.h:46-53
+            float h1 = kc.c[0] * dxx;
+            h1 = __builtin_fmaf(kc.c[1], dyy, h1);
+            h1 = __builtin_fmaf(kc.c[2], dzz, h1);
+            h1 = __builtin_fmaf(kc.c[3], dxy, h1);
+            h1 = __builtin_fmaf(kc.c[5], dxz, h1);
+            h1 = __builtin_fmaf(kc.c[4], dyz, h1);
+            const float h2 = dxx + dyy + dzz - h1;
+            // lanes 0-31 receive h1q, lanes 32-63 receive h2p
+            const auto sw = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, h1),
+                                                             __builtin_bit_cast(uint32_t, h2), false, false);
+            const float other = __builtin_bit_cast(float, fld ? sw[0] : sw[1]);
+            // rhsp = v2px h2p + v2pz h1q + v2sz (h1p - h1q)
+            // rhsq = v2pn h2p + v2pz h1q - v2sz (h2p - h2q) = ... + v2sz (h2q - h2p)
+            // with h2p, h1q selected per half (original/sample.h:75-81, same roundings)
+            const float h2p = fld ? other : h2, h1q = fld ? h1 : other;
+            float rhs = (fld ? kc.c[9] : kc.c[6]) * h2p;
+            rhs = __builtin_fmaf(kc.c[7], h1q, rhs);
+            rhs = __builtin_fmaf(kc.c[8], (fld ? h2 : h1) - other, rhs);
+            const float cur = qz[4];
+            const float prev = fld ? vqp : vpp;
+            const float base = 2.0f * cur - prev;
+            float nw;
+            if constexpr (CM == CM_F32) nw = __builtin_fmaf(rhs * a.dt, a.dt, base);
+            else nw = base + rhs;
+            if (z == rec_z) {
+                if (fld == 0 && active)
+                    ctl.trace[(y - a.bord) * (a.sx - 2 * a.bord) + (x - a.bord)] = nw;
+            }
+            if (z == src_z) {
+                if (col == src_col && active) nw += ctl.src_add;
+            }
+            bstore32(r_pp, vop, so_o, nw);
+            bstore32(r_qp, voq, so_o, nw);
+        }
+        if constexpr (GEN) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                qc_[k] = qc_[k + 1];
+                qx_[k] = qx_[k + 1];
+                qy_[k] = qy_[k + 1];
+            }
+        }
+        r6 = r6n;
+        r5 = r5n;
+        __syncthreads();
+    };
+
+    // planes 0-3: queues only; 4-7: queues + Dx halo planes (when an output plane);
+    // ring phase of plane i is (i+1) % 9
+    step(ic<1>{}, ic<0>{}, ic<0>{}, ic<0>{}, 0);
+    step(ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, 1);
+    step(ic<3>{}, ic<0>{}, ic<0>{}, ic<0>{}, 2);
+    step(ic<4>{}, ic<0>{}, ic<0>{}, ic<0>{}, 3);
+    step(ic<5>{}, ic<2>{}, ic<0>{}, ic<0>{}, 4);
+    step(ic<6>{}, ic<2>{}, ic<0>{}, ic<0>{}, 5);
+    step(ic<7>{}, ic<2>{}, ic<0>{}, ic<0>{}, 6);
+    step(ic<8>{}, ic<2>{}, ic<0>{}, ic<0>{}, 7);
+    // steady state: groups of 9 planes, each writing Dx halos and an output plane
+    int i = 8;
+    for (; i + 8 < nz + 4; i += 9) {
+        unroll_phases([&](auto P) { step(P, ic<1>{}, ic<1>{}, ic<0>{}, i + decltype(P)::value); },
+                      std::make_integer_sequence<int, 9>{});
+    }
+    // the rest (ring phase 0 here since i = 8 mod 9), queues shifted after each plane
+    for (; i <= nz + 7; i++) step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, i);
+}
+
+// Minimum waves per SIMD requested from the register allocator (0: compiler default).
+#ifndef FL_STEP_WAVES
+#define FL_STEP_WAVES 4
+#endif
+#if FL_STEP_WAVES > 0
+#define FL_STEP_ATTR __attribute__((amdgpu_waves_per_eu(FL_STEP_WAVES)))
+#else
+#define FL_STEP_ATTR
+#endif
+
+template <int TY, int CM, bool TWOPASS>
+__global__ __launch_bounds__(64 * TY) FL_STEP_ATTR void tti_step(const StepArgs a)
+{
+    __shared__ StepLds<TY> s;
+    if (a.it_next && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0)
+        *a.it_next = (a.it_dev ? *a.it_dev : 0) + 1;
+    const StepCtl ctl = step_ctl(a);
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    if (t >= a.nblk) return;
+    const int tix = t % a.ntx;
+    const int rest = t / a.ntx;
+    step_tile<TY, CM, TWOPASS>(a, ctl, tix, rest % a.nty, rest / a.nty, s);
+}
+
+// Shot of a batched run (device copy of the host descriptor).
+struct ShotDev {
+    float *p[2], *q[2];
+    const void *coef;   // coefficient records
+    float *trace;
+    long long src_idx;
+    int sx, sy, sz, rec_iz;
+    int ntx, nty, item0, nitems;  // tiles along x, y; first work item; item count
+};
+
+struct BatchArgs {
+    const ShotDev *shots;
+    int nshots, nitems, parity, zt;
+    float dxxinv, dyyinv, dzzinv, dxinv, dyinv, dzinv, dt;
+    int bord;
+    int it_base;
+    const int *it_dev;
+    int *it_next;
+    const float *src_tab;
+    int src_n, rec_every, rec_slots;
+};
+
+// Source of the first step of a batched run: every shot's pc, qc at its source cell.
+__global__ void batch_insert(const ShotDev *shots, int nshots, int parity, float s)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nshots) return;
+    const ShotDev &sh = shots[i];
+    sh.p[1 - parity][sh.src_idx] += s;
+    sh.q[1 - parity][sh.src_idx] += s;
+}
+
+// Batched shots: a persistent grid walks the (shot, tile) work list.
+template <int TY, int CM>
+__global__ __launch_bounds__(64 * TY) FL_STEP_ATTR void tti_step_batch(const BatchArgs b)
+{
+    __shared__ StepLds<TY> s;
+    const int k = *b.it_dev;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *b.it_next = k + 1;
+    const int it = b.it_base + 1 + k;
+    for (int t = xcd_remap(blockIdx.x, gridDim.x); t < b.nitems; t += gridDim.x) {
+        int lo = 0, hi = b.nshots - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (b.shots[mid].item0 <= t) lo = mid; else hi = mid - 1;
+        }
+        const ShotDev &sh = b.shots[lo];
+        StepArgs a;
+        a.pp = sh.p[b.parity];
+        a.pc = sh.p[1 - b.parity];
+        a.qp = sh.q[b.parity];
+        a.qc = sh.q[1 - b.parity];
+        a.coef = sh.coef;
+        a.dvol = nullptr;
+        a.dxxinv = b.dxxinv; a.dyyinv = b.dyyinv; a.dzzinv = b.dzzinv;
+        a.dxinv = b.dxinv; a.dyinv = b.dyinv; a.dzinv = b.dzinv; a.dt = b.dt;
+        a.sx = sh.sx; a.sy = sh.sy; a.sz = sh.sz; a.bord = b.bord;
+        a.ntx = sh.ntx; a.nty = sh.nty; a.zt = b.zt; a.nblk = sh.nitems;
+        a.rec_iz = sh.rec_iz;
+        a.src_idx = sh.src_idx;
+        StepCtl ctl;
+        ctl.do_src = b.src_tab && (k + 1) < b.src_n;
+        ctl.src_add = ctl.do_src ? b.src_tab[k + 1] : 0.0f;
+        ctl.trace = nullptr;
+        if (sh.trace && b.rec_every > 0 && it % b.rec_every == 0) {
+            const int slot = it / b.rec_every - 1;
+            if (slot >= 0 && slot < b.rec_slots)
+                ctl.trace = sh.trace + (long long)slot * (sh.sx - 2 * b.bord) * (sh.sy - 2 * b.bord);
+        }
+        const int l = t - sh.item0;
+        const int tix = l % sh.ntx;
+        const int rest = l / sh.ntx;
+        step_tile<TY, CM, false>(a, ctl, tix, rest % sh.nty, rest / sh.nty, s);
+    }
+}
+
+}  // namespace fl

@@ -304,3 +304,17 @@ void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const fl
         out[5][i] = fl_f64_to_f16((double)phi[i] - pi);
     }
 }
+
+void fl_interleave32(int64_t n, int nk, const float *const *src, float *dst)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++)
+        for (int k = 0; k < nk; k++) dst[i * nk + k] = src[k][i];
+}
+
+void fl_interleave16(int64_t n, int nk, const uint16_t *const *src, uint16_t *dst)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++)
+        for (int k = 0; k < nk; k++) dst[i * nk + k] = src[k][i];
+}

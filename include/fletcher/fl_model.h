@@ -66,6 +66,10 @@ void fl_pack_coefs16(int64_t n, const float *vpz, const float *vsv, const float 
 void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, double dt,
                          uint16_t *const *out);
+/* Cell records: dst[i*nk + k] = src[k][i] (the device keeps the coefficients of a cell
+ * together, one record per cell). */
+void fl_interleave32(int64_t n, int nk, const float *const *src, float *dst);
+void fl_interleave16(int64_t n, int nk, const uint16_t *const *src, uint16_t *dst);
 /* Single conversions (round to nearest even), exported for tests. */
 uint16_t fl_f64_to_f16(double v);
 uint16_t fl_f64_to_bf16(double v);

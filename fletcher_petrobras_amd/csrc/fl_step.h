@@ -9,8 +9,8 @@
 //   registers  9-deep z queues of c = pc|qc, Dx and Dy over planes f-8 .. f (Dzz, Dxz, Dyz
 //              of the output plane z = f-4: der1der1lm's 9-plane ring,
 //              der1der1lm/zTileSize.h:4-7 and sample.h:147-160, held in registers).  The
-//              ring is addressed statically: the steady loop is unrolled 9 times, one copy
-//              per ring phase, so planes advance without register moves.
+//              ring (10 slots) is addressed statically: the steady loop is unrolled 10
+//              times, one copy per ring phase, so planes advance without register moves.
 //   LDS        a ring of 6 staged planes of pc, qc (tile + 4-cell halo): plane f for Dx, Dy
 //              and the queues, plane z = f-4 for Dxx, Dyy, plane f+1 being written;
 //              a ring of 5 planes of Dx over the tile rows plus the 4-row y halo, from which
@@ -41,6 +41,11 @@ __device__ __forceinline__ float ld_f32(const void *base, uint32_t off)
 {
     return *(const float *)((const char *)base + off);
 }
+
+// 1: the steady steps load the next output plane's coefficients and pp/qp one step ahead.
+#ifndef FL_PREFETCH
+#define FL_PREFETCH 1
+#endif
 
 // Lane byte offset beyond every buffer range used here (spans are < 2^31 bytes).
 constexpr uint32_t FL_OOB = 0x80000000u;
@@ -73,8 +78,8 @@ __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, uint32_t voff
 {
     if constexpr (CM == CM_F32) {
         const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
-        const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16u, 0, 0);
-        const auto d = __builtin_amdgcn_raw_buffer_load_b64(r, voff + 32u, 0, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 16, 0);
+        const auto d = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 32, 0);
         c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
         c.w[4] = b[0]; c.w[5] = b[1]; c.w[6] = b[2]; c.w[7] = b[3];
         c.w[8] = d[0]; c.w[9] = d[1];
@@ -84,7 +89,7 @@ __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, uint32_t voff
     } else {
         const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
         c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
-        c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, voff + 16u, 0, 0);
+        c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, voff, 16, 0);
     }
 }
 
@@ -144,6 +149,9 @@ struct StepLds {
 template <int V>
 using ic = std::integral_constant<int, V>;
 
+// An empty register constraint: the value is materialised in a VGPR at this point.
+__device__ __forceinline__ void fl_pin(float &x) { asm volatile("" : "+v"(x)); }
+
 template <class F, int... P>
 __device__ __forceinline__ void unroll_phases(F &&fn, std::integer_sequence<int, P...>)
 {
@@ -164,10 +172,10 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     const int zlo = a.bord + zc * a.zt;
     const int zhi = min(zlo + a.zt, a.sz - a.bord);
     const int nz = zhi - zlo, f0 = zlo - 4;
-    const int lane = threadIdx.x, ly = threadIdx.y;
-    const int wv = __builtin_amdgcn_readfirstlane(ly);
+    const int lane = threadIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // tile row (wave-uniform)
     const int fld = lane >> 5, cx = lane & 31;
-    const int x = x0 + cx, y = y0 + ly;
+    const int x = x0 + cx, y = y0 + wv;
     const bool active = (x >= a.bord) && (x < a.sx - a.bord) && (y < a.sy - a.bord);
     const uint32_t sxy = (uint32_t)a.sx * (uint32_t)a.sy;
     const uint32_t pb4 = sxy * 4u, pbr = sxy * (uint32_t)REC;
@@ -197,20 +205,21 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     // staging: element e' = wave*64 + k*NT + lane of [0, 2*HTP); p for e' < HTP, q above;
     // the field of a (wave, k) pair is uniform
     uint32_t hoff[NLD];
-    int hl[NLD];
+    int hs[NLD];  // LDS element of lane 0 (wave-uniform); lane l stores element hs + l
     bool hq[NLD], hk[NLD];
 #pragma unroll
     for (int k = 0; k < NLD; k++) {
         const int ew = wv * 64 + k * NT;
         hk[k] = ew < 2 * HTP;
         hq[k] = ew >= HTP;
-        const int ee = ew + lane - (hq[k] ? HTP : 0);
+        const int e0 = ew - (hq[k] ? HTP : 0);  // element within the field, lane 0
+        const int ee = e0 + lane;
         const bool inb = hk[k] && (HT == HTP || ee < HT);
         const int r = ee / HX, c = ee - r * HX;
         const int gy = y0 - 4 + r, gx = x0 - 4 + c;
         const bool ok = inb && gx >= 0 && gx < a.sx && gy < a.sy;
         hoff[k] = ok ? (uint32_t)(gy * a.sx + gx) * 4u : FL_OOB;
-        hl[k] = (hq[k] ? HT : 0) + ee;
+        hs[k] = (hq[k] ? HT : 0) + e0;
     }
     float pre[NLD];
     auto fetch = [&](int i) {  // raw loads of input plane f0 + i
@@ -222,13 +231,13 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     auto stage = [&](float *T) {
 #pragma unroll
         for (int k = 0; k < NLD; k++)
-            if (hk[k] && (HT == HTP || hl[k] < (hq[k] ? 2 * HT : HT))) T[hl[k]] = pre[k];
+            if (hk[k] && (HT == HTP || hs[k] + lane < (hq[k] ? 2 * HT : HT))) T[hs[k] + lane] = pre[k];
     };
 
     // lane offsets in the LDS planes
     // (window corners, so that every stencil read is a non-negative immediate offset)
-    const int ow = fld * HT + ly * HX + cx;  // 9 x 9 window of the own cell in a staged plane
-    const int oxw = fld * XT + ly * TX + cx; // 9-row column of the own cell in a Dx plane
+    const int ow = fld * HT + wv * HX + cx;  // 9 x 9 window of the own cell in a staged plane
+    const int oxw = fld * XT + wv * TX + cx; // 9-row column of the own cell in a Dx plane
     constexpr int CW = 4 * HX + 4, CX = 4 * TX;  // the own cell within them
     int ohr[NH], ohx[NH];                              // y-halo rows of Dx done by this wave
     bool hrow[NH];
@@ -246,9 +255,14 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     const float *dvx = TWOPASS ? a.dvol + (fld ? 2 * nvol : 0) : nullptr;
     const float *dvy = TWOPASS ? a.dvol + (fld ? 3 * nvol : nvol) : nullptr;
 
-    float qc_[9], qx_[9], qy_[9];  // own field c, Dx, Dy of planes f-8..f (ring)
+    // own field c, Dx, Dy of planes f-8..f: ring of 10 slots (plane j in slot j % 10 in
+    // the statically addressed steps; 10 keeps the step parity of every slot fixed)
+    float qc_[10], qx_[10], qy_[10];
 #pragma unroll
-    for (int k = 0; k < 9; k++) qc_[k] = qx_[k] = qy_[k] = 0.0f;
+    for (int k = 0; k < 10; k++) qc_[k] = qx_[k] = qy_[k] = 0.0f;
+    // output-plane streams loaded one step ahead (set = parity of the consuming step)
+    CoefRec<CM> kp[2];
+    float ppp[2] = {0.0f, 0.0f}, qpp[2] = {0.0f, 0.0f};
 
     // prologue: plane f0 into ring slot 0, loads of plane f0+1 in flight
     fetch(0);
@@ -257,12 +271,20 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     __syncthreads();
 
     int r6 = 0, r5 = 0;  // LDS ring slots of plane f0 + i: i % 6, i % 5
-    // one input plane: PH ring phase, XW 0/1/2 = Dx halo planes never/always/if i < nz+4,
-    // OUT output plane z = f-4, GEN generic copy (phase 0, queues shifted afterwards)
-    auto step = [&](auto PHc, auto XWc, auto OUTc, auto GENc, const int i) {
+    // one input plane i (f = f0 + i):
+    //   PH   queue slot of plane i (static steps: i % 10)
+    //   XW   Dx halo planes 0 never / 1 always / 2 if i < nz+4 (f is an output plane)
+    //   OUT  output plane z = f-4
+    //   GEN  generic copy: plane i-8+k in slot k, queues shifted afterwards
+    //   PF   bit 0: this step's output streams were loaded by the previous step into set
+    //        i % 2; bits 1-2: load those of the next step into set (i+1) % 2 (2 always,
+    //        4 if `pf_next`)
+    auto step = [&](auto PHc, auto XWc, auto OUTc, auto GENc, auto PFc, const int i, const bool pf_next) {
         constexpr int PH = decltype(PHc)::value;
         constexpr int XW = decltype(XWc)::value;
         constexpr bool OUT = decltype(OUTc)::value != 0, GEN = decltype(GENc)::value != 0;
+        constexpr int PF = decltype(PFc)::value;
+        constexpr int SET = PH & 1;  // = i % 2 in the static steps
         const int f = f0 + i;
         const int z = f - 4;
         const int r6n = r6 == 5 ? 0 : r6 + 1, r5n = r5 == 4 ? 0 : r5 + 1;
@@ -275,7 +297,11 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         // streams of the output plane, kept raw until used
         CoefRec<CM> kr;
         float vpp = 0.0f, vqp = 0.0f;
-        if constexpr (OUT) {
+        if constexpr (OUT && (PF & 1)) {
+            kr = kp[SET];
+            vpp = ppp[SET];
+            vqp = qpp[SET];
+        } else if constexpr (OUT) {
             load_rec<CM>(span_rsrc(cpl, pbr), vor, kr);
             cpl += pbr;
             vpp = __builtin_bit_cast(float, bload32(r_pp, vop, so_o));
@@ -298,7 +324,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             }
         }
         const float *cp = Tf + ow + CW;
-        const float nc = cp[0];
+        float nc = cp[0];
         if constexpr (!TWOPASS) {
             nx = der1_lds(cp, 1, a.dxinv);
             ny = der1_lds(cp, HX, a.dyinv);
@@ -312,7 +338,12 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
                     else Xf[ohx[m]] = der1_lds(Tf + ohr[m] + 4, 1, a.dxinv);
                 }
         }
-        constexpr int ins = GEN ? 8 : (PH + 8) % 9;
+        // formed here, in this step: keeps the compiler from carrying the raw LDS values to
+        // the step that first uses the queue entry instead
+        fl_pin(nc);
+        fl_pin(nx);
+        fl_pin(ny);
+        constexpr int ins = GEN ? 8 : PH;
         qc_[ins] = nc;
         qx_[ins] = nx;
         qy_[ins] = ny;
@@ -329,13 +360,13 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         if (i + 2 <= nz + 7) fetch(i + 2);
 
         if constexpr (OUT) {
-            constexpr int b = GEN ? 0 : PH;
+            constexpr int b = GEN ? 0 : (PH + 2) % 10;  // slot of plane i-8
             float qz[9], qzx[9], qzy[9];
 #pragma unroll
             for (int k = 0; k < 9; k++) {
-                qz[k] = qc_[(b + k) % 9];
-                qzx[k] = qx_[(b + k) % 9];
-                qzy[k] = qy_[(b + k) % 9];
+                qz[k] = qc_[(b + k) % 10];
+                qzx[k] = qx_[(b + k) % 10];
+                qzy[k] = qy_[(b + k) % 10];
             }
             const float dzz = der2_q(qz, a.dzzinv);
             const float dxz = der1_q(qzx, a.dzinv);
@@ -377,6 +408,15 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             bstore32(r_pp, vop, so_o, nw);
             bstore32(r_qp, voq, so_o, nw);
         }
+        if constexpr ((PF & 2) != 0 || (PF & 4) != 0) {
+            if ((PF & 2) || pf_next) {
+                const uint32_t so_n = (uint32_t)(i - 7) * pb4;
+                load_rec<CM>(span_rsrc(cpl, pbr), vor, kp[SET ^ 1]);
+                cpl += pbr;
+                ppp[SET ^ 1] = __builtin_bit_cast(float, bload32(r_pp, vop, so_n));
+                qpp[SET ^ 1] = __builtin_bit_cast(float, bload32(r_qp, voq, so_n));
+            }
+        }
         if constexpr (GEN) {
 #pragma unroll
             for (int k = 0; k < 8; k++) {
@@ -390,24 +430,32 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         __syncthreads();
     };
 
-    // planes 0-3: queues only; 4-7: queues + Dx halo planes (when an output plane);
-    // ring phase of plane i is (i+1) % 9
-    step(ic<1>{}, ic<0>{}, ic<0>{}, ic<0>{}, 0);
-    step(ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, 1);
-    step(ic<3>{}, ic<0>{}, ic<0>{}, ic<0>{}, 2);
-    step(ic<4>{}, ic<0>{}, ic<0>{}, ic<0>{}, 3);
-    step(ic<5>{}, ic<2>{}, ic<0>{}, ic<0>{}, 4);
-    step(ic<6>{}, ic<2>{}, ic<0>{}, ic<0>{}, 5);
-    step(ic<7>{}, ic<2>{}, ic<0>{}, ic<0>{}, 6);
-    step(ic<8>{}, ic<2>{}, ic<0>{}, ic<0>{}, 7);
-    // steady state: groups of 9 planes, each writing Dx halos and an output plane
+    // planes 0-3: queues only; 4-7: queues + Dx halo planes (when an output plane); plane 7
+    // loads the streams of the first output plane when a static group follows
+    const bool grp0 = 8 + 9 < nz + 4;
+    step(ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 0, false);
+    step(ic<1>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 1, false);
+    step(ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 2, false);
+    step(ic<3>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 3, false);
+    step(ic<4>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, 4, false);
+    step(ic<5>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, 5, false);
+    step(ic<6>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, 6, false);
+    step(ic<7>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<FL_PREFETCH ? 4 : 0>{}, 7, grp0);
+    // steady state: groups of 10 planes (i = 8 mod 10 at their start), each writing Dx
+    // halos and an output plane, with the output streams one step ahead
     int i = 8;
-    for (; i + 8 < nz + 4; i += 9) {
-        unroll_phases([&](auto P) { step(P, ic<1>{}, ic<1>{}, ic<0>{}, i + decltype(P)::value); },
-                      std::make_integer_sequence<int, 9>{});
+    for (; i + 9 < nz + 4; i += 10) {
+        const bool more = i + 10 + 9 < nz + 4;
+        unroll_phases(
+            [&](auto P) {
+                constexpr int p = decltype(P)::value;
+                constexpr int pf = FL_PREFETCH ? (1 | (p < 9 ? 2 : 4)) : 0;
+                step(ic<(8 + p) % 10>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<pf>{}, i + p, more);
+            },
+            std::make_integer_sequence<int, 10>{});
     }
-    // the rest (ring phase 0 here since i = 8 mod 9), queues shifted after each plane
-    for (; i <= nz + 7; i++) step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, i);
+    // the rest, queues shifted after each plane
+    for (; i <= nz + 7; i++) step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, i, false);
 }
 
 // Minimum waves per SIMD requested from the register allocator (0: compiler default).

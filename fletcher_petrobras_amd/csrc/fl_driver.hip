@@ -60,13 +60,15 @@ struct Tiles {
 
 struct Device {
     bool init = false;
-    int sx = 0, sy = 0, sz = 0, bord = 0;
+    int sx = 0, sy = 0, sz = 0, bord = 0, px = 0;
     float dx = 0, dy = 0, dz = 0, dt = 0;
-    long long n = 0;
-    float *f[4] = {nullptr, nullptr, nullptr, nullptr};  // pp pc qp qc (current roles)
-    void *coef = nullptr;
+    long long n = 0;        // cells of the reference layout, sx*sy*sz
+    long long np = 0;       // cells of the pitched device layout, px*sy*sz
+    float *lv[2] = {nullptr, nullptr};  // time levels of (p, q) pairs
+    int lc = 1;             // level holding pc, qc; lv[1 - lc] holds pp, qp
+    void *coef = nullptr;   // coefficient records, pitched like the levels
     int cm = CM_F32;
-    float *dvol = nullptr;  // two-pass first-derivative volumes (4n)
+    float *dvx = nullptr, *dvy = nullptr;  // two-pass (pDx, qDx), (pDy, qDy) pairs
     float *trace = nullptr;
     int rec_iz = -1, rec_every = 0, rec_slots = 0;
     int *ctr = nullptr;     // step counters for captured runs (2 ints)
@@ -87,11 +89,24 @@ static hipEvent_t g_reg0 = nullptr, g_reg1 = nullptr;
 // tile width of the step kernel (columns per wave and field)
 static int tile_x() { return 32; }
 
-// Deepest z chunk whose pc/qc span (zt + 8 planes) fits a buffer range (< 2^31 bytes).
-static int max_zt(int sx, int sy)
+// Row pitch of the device layout: 16 cells = one 128-byte line of (p, q) pairs.
+static int pitch_of(int sx) { return (sx + 15) / 16 * 16; }
+
+// Deepest z chunk whose level span (zt + 8 pitched planes of pairs) fits a buffer range
+// (< 2^31 bytes); 0 when not even one chunk does or a record plane does not.
+static int max_zt(int px, int sy)
 {
-    const long long pb = (long long)sx * sy * 4;
+    const long long pb = (long long)px * sy * 8;
+    if ((long long)px * sy * 40 >= 0x7fffffffll) return 0;
     return (int)std::max<long long>(0, (0x7fffffffll / pb) - 8);
+}
+
+// Reference-layout index -> pitched cell index.
+static long long pitched_index(long long i, int sx, int sy, int px)
+{
+    const long long sxy = (long long)sx * sy;
+    const long long iz = i / sxy, r = i - iz * sxy, iy = r / sx, ix = r - iy * sx;
+    return (iz * sy + iy) * px + ix;
 }
 
 static int ensure_stream()
@@ -133,12 +148,14 @@ extern "C" int fl_bytes_per_cell(void)
 // values in precomp.h order (FP32) or the 16-bit forms, consecutive (record of
 // coef_count * coef_elem bytes); z slabs of at most 64 MiB per array on the host.
 static int upload_coefs(const float *vpz, const float *vsv, const float *eps, const float *del,
-                        const float *phi, const float *theta, float dt, int cm, void *dst,
-                        long long n, long long sxy, int sz)
+                        const float *phi, const float *theta, float dt, int cm, void *dst, int sx,
+                        int sy, int sz, int px)
 {
-    (void)n;
+    const long long sxy = (long long)sx * sy;
     const int slab = std::max(1, (int)std::min<long long>(sz, (64ll << 20) / (sxy * 4)));
     const int nk = coef_count(cm);
+    const size_t rec = (size_t)nk * coef_elem(cm);
+    HIPCHK(hipMemset(dst, 0, rec * px * sy * sz));  // pitch padding
     std::vector<float> f32, r32;
     std::vector<uint16_t> h16, r16;
     for (int z0 = 0; z0 < sz; z0 += slab) {
@@ -151,7 +168,8 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
             for (int k = 0; k < 10; k++) c[k] = f32.data() + k * m;
             fl_precompute_f32(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, c);
             fl_interleave32(m, 10, c, r32.data());
-            HIPCHK(hipMemcpy((float *)dst + off * 10, r32.data(), m * 40, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy2D((char *)dst + (size_t)z0 * sy * px * rec, px * rec, r32.data(), sx * rec,
+                               sx * rec, (size_t)sy * (z1 - z0), hipMemcpyHostToDevice));
         } else {
             h16.resize((size_t)nk * m);
             r16.resize((size_t)nk * m);
@@ -163,7 +181,8 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
                 fl_pack_coefs16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
                                 cm == CM_BF16, c);
             fl_interleave16(m, nk, c, r16.data());
-            HIPCHK(hipMemcpy((uint16_t *)dst + off * nk, r16.data(), m * nk * 2, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy2D((char *)dst + (size_t)z0 * sy * px * rec, px * rec, r16.data(), sx * rec,
+                               sx * rec, (size_t)sy * (z1 - z0), hipMemcpyHostToDevice));
         }
     }
     return FL_OK;
@@ -179,7 +198,7 @@ static int default_zt(int sx, int sy, int sz, int bord, int ty)
     int nzc = (int)std::max<long long>(1, (target + per_plane - 1) / per_plane);
     int zt = (nzu + nzc - 1) / nzc;
     zt = std::max(zt, 16);
-    return std::min(std::min(zt, nzu), max_zt(sx, sy));
+    return std::min(std::min(zt, nzu), max_zt(pitch_of(sx), sy));
 }
 
 static void fill_geom(StepArgs &a, int sx, int sy, int sz, int bord, int ty, int zt)
@@ -222,9 +241,12 @@ static StepArgs base_args()
 {
     StepArgs a;
     memset(&a, 0, sizeof(a));
-    a.pp = D.f[0]; a.pc = D.f[1]; a.qp = D.f[2]; a.qc = D.f[3];
+    a.prev = D.lv[1 - D.lc];
+    a.cur = D.lv[D.lc];
     a.coef = D.coef;
-    a.dvol = D.dvol;
+    a.dvx = D.dvx;
+    a.dvy = D.dvy;
+    a.px = D.px;
     a.dxxinv = 1.0f / (D.dx * D.dx);
     a.dyyinv = 1.0f / (D.dy * D.dy);
     a.dzzinv = 1.0f / (D.dz * D.dz);
@@ -232,7 +254,7 @@ static StepArgs base_args()
     a.dyinv = 1.0f / D.dy;
     a.dzinv = 1.0f / D.dz;
     a.dt = D.dt;
-    const int zt = g_tiles.zt > 0 ? std::min(std::min(g_tiles.zt, D.sz - 2 * D.bord), max_zt(D.sx, D.sy))
+    const int zt = g_tiles.zt > 0 ? std::min(std::min(g_tiles.zt, D.sz - 2 * D.bord), max_zt(D.px, D.sy))
                                   : default_zt(D.sx, D.sy, D.sz, D.bord, g_tiles.ty);
     fill_geom(a, D.sx, D.sy, D.sz, D.bord, g_tiles.ty, zt);
     a.src_idx = -1;
@@ -243,18 +265,42 @@ static StepArgs base_args()
     return a;
 }
 
-static void swap_levels()
-{
-    std::swap(D.f[0], D.f[1]);
-    std::swap(D.f[2], D.f[3]);
-}
+static void swap_levels() { D.lc ^= 1; }
 
 static int ensure_dvol()
 {
-    if (D.dvol) return FL_OK;
-    int rc = dev_alloc((void **)&D.dvol, (size_t)4 * D.n * 4, &D.bytes);
+    if (D.dvx) return FL_OK;
+    int rc = dev_alloc((void **)&D.dvx, (size_t)2 * D.np * 4, &D.bytes);
+    if (!rc) rc = dev_alloc((void **)&D.dvy, (size_t)2 * D.np * 4, &D.bytes);
     if (rc) return rc;
-    HIPCHK(hipMemsetAsync(D.dvol, 0, (size_t)4 * D.n * 4, g_stream));
+    HIPCHK(hipMemsetAsync(D.dvx, 0, (size_t)2 * D.np * 4, g_stream));
+    HIPCHK(hipMemsetAsync(D.dvy, 0, (size_t)2 * D.np * 4, g_stream));
+    return FL_OK;
+}
+
+// One field (fld 0 p, 1 q) of a level from / to a reference-layout host volume, through a
+// linear device buffer.
+static int level_xfer(float *lv, int fld, float *host, bool put, int sx, int sy, int sz, int px)
+{
+    const long long n = (long long)sx * sy * sz;
+    float *tmp = nullptr;
+    long long dummy = 0;
+    int rc = dev_alloc((void **)&tmp, (size_t)n * 4, &dummy);
+    if (rc) return rc;
+    hipError_t e = hipStreamSynchronize(g_stream);
+    const dim3 g((sx + 255) / 256, sy, sz);
+    if (put) {
+        if (e == hipSuccess) e = hipMemcpy(tmp, host, (size_t)n * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) hipLaunchKernelGGL(field_pack, g, dim3(256), 0, g_stream, tmp, lv, fld, sx, sy, px);
+        if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    } else {
+        hipLaunchKernelGGL(field_unpack, g, dim3(256), 0, g_stream, lv, tmp, fld, sx, sy, px);
+        if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+        if (e == hipSuccess) e = hipMemcpy(host, tmp, (size_t)n * 4, hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(FL_ERR_HIP, "field transfer: %s", hipGetErrorString(e));
     return FL_OK;
 }
 
@@ -270,17 +316,19 @@ static int launch_step(StepArgs a, int it_host, hipStream_t s)
     if (g_variant == FL_VARIANT_TWOPASS) {
         int rc = ensure_dvol();
         if (rc) return rc;
-        a.dvol = D.dvol;
+        a.dvx = D.dvx;
+        a.dvy = D.dvy;
         dim3 g1((D.sx + 63) / 64, (D.sy + 3) / 4, D.sz);
-        hipLaunchKernelGGL(der1_pass, g1, dim3(64, 4), 0, s, a.pc, a.qc, D.dvol, D.sx, D.sy, D.sz,
+        hipLaunchKernelGGL(der1_pass, g1, dim3(64, 4), 0, s, a.cur, D.dvx, D.dvy, D.sx, D.sy, D.px,
                            D.bord, a.dxinv, a.dyinv);
         return launch_steps(a, g_tiles.ty, CM_F32, true, s);
     }
     if (g_variant == FL_VARIANT_DIRECT) {
         DirectArgs d;
-        d.pp = a.pp; d.pc = a.pc; d.qp = a.qp; d.qc = a.qc;
+        d.prev = a.prev;
+        d.cur = a.cur;
         d.coef = (const float *)D.coef;
-
+        d.px = D.px;
         d.dxxinv = a.dxxinv; d.dyyinv = a.dyyinv; d.dzzinv = a.dzzinv;
         d.dxyinv = 1.0f / (D.dx * D.dy);
         d.dxzinv = 1.0f / (D.dx * D.dz);
@@ -320,12 +368,11 @@ static int take_events(hipEvent_t *e0, hipEvent_t *e1)
 static void free_device()
 {
     if (g_stream) (void)hipStreamSynchronize(g_stream);
-    for (int k = 0; k < 4; k++) {
-        if (D.f[k]) (void)hipFree(D.f[k]);
-        D.f[k] = nullptr;
-    }
+    for (int k = 0; k < 2; k++)
+        if (D.lv[k]) (void)hipFree(D.lv[k]);
     if (D.coef) (void)hipFree(D.coef);
-    if (D.dvol) (void)hipFree(D.dvol);
+    if (D.dvx) (void)hipFree(D.dvx);
+    if (D.dvy) (void)hipFree(D.dvy);
     if (D.trace) (void)hipFree(D.trace);
     if (D.ctr) (void)hipFree(D.ctr);
     if (D.src_tab) (void)hipFree(D.src_tab);
@@ -344,7 +391,7 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
         return fail(FL_ERR_ARG, "spacings and dt must be positive");
     if (!vpz || !vsv || !eps || !del || !phi || !theta)
         return fail(FL_ERR_ARG, "model arrays must not be NULL");
-    if (max_zt(sx, sy) < 1 || (long long)sx * sy * 40 >= 0x7fffffffll)
+    if (max_zt(pitch_of(sx), sy) < 1)
         return fail(FL_ERR_ARG, "xy plane %dx%d too large for 32-bit plane offsets", sx, sy);
     int rc = ensure_stream();
     if (rc) return rc;
@@ -353,19 +400,28 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
     D.sx = sx; D.sy = sy; D.sz = sz; D.bord = bord;
     D.dx = dx; D.dy = dy; D.dz = dz; D.dt = dt;
     D.n = n;
+    D.px = pitch_of(sx);
+    D.np = (long long)D.px * sy * sz;
+    D.lc = 1;
     D.cm = g_precision;
-    for (int k = 0; k < 4; k++) {
-        if ((rc = dev_alloc((void **)&D.f[k], (size_t)n * 4, &D.bytes))) { free_device(); return rc; }
+    for (int k = 0; k < 2; k++) {
+        if ((rc = dev_alloc((void **)&D.lv[k], (size_t)2 * D.np * 4, &D.bytes))) { free_device(); return rc; }
+        hipError_t e = hipMemsetAsync(D.lv[k], 0, (size_t)2 * D.np * 4, g_stream);
+        if (e != hipSuccess) { free_device(); return fail(FL_ERR_HIP, "field clear: %s", hipGetErrorString(e)); }
     }
+    // pp, qp into the previous level, pc, qc into the current one
     const float *host[4] = {pp, pc, qp, qc};
     for (int k = 0; k < 4; k++) {
-        hipError_t e = host[k] ? hipMemcpy(D.f[k], host[k], (size_t)n * 4, hipMemcpyHostToDevice)
-                               : hipMemsetAsync(D.f[k], 0, (size_t)n * 4, g_stream);
-        if (e != hipSuccess) { free_device(); return fail(FL_ERR_HIP, "field upload: %s", hipGetErrorString(e)); }
+        if (!host[k]) continue;
+        float *lv = (k & 1) ? D.lv[D.lc] : D.lv[1 - D.lc];
+        if ((rc = level_xfer(lv, k >> 1, const_cast<float *>(host[k]), true, sx, sy, sz, D.px))) {
+            free_device();
+            return rc;
+        }
     }
-    const size_t cbytes = (size_t)coef_count(D.cm) * n * coef_elem(D.cm);
+    const size_t cbytes = (size_t)coef_count(D.cm) * D.np * coef_elem(D.cm);
     if ((rc = dev_alloc(&D.coef, cbytes, &D.bytes))) { free_device(); return rc; }
-    if ((rc = upload_coefs(vpz, vsv, eps, del, phi, theta, dt, D.cm, D.coef, n, (long long)sx * sy, sz))) {
+    if ((rc = upload_coefs(vpz, vsv, eps, del, phi, theta, dt, D.cm, D.coef, sx, sy, sz, D.px))) {
         free_device();
         return rc;
     }
@@ -426,8 +482,8 @@ static int insert_src(int iSource, float src)
     if (!D.init) return fail(FL_ERR_STATE, "DRIVER_InsertSource before DRIVER_Initialize");
     if (iSource < 0 || (long long)iSource >= D.n)
         return fail(FL_ERR_ARG, "iSource=%d outside the %lld-cell grid", iSource, D.n);
-    hipLaunchKernelGGL(insert_source, dim3(1), dim3(64), 0, g_stream, D.f[1], D.f[3],
-                       (long long)iSource, src);
+    hipLaunchKernelGGL(insert_source, dim3(1), dim3(64), 0, g_stream, D.lv[D.lc],
+                       pitched_index(iSource, D.sx, D.sy, D.px), src);
     HIPCHK(hipGetLastError());
     if (g_sync) HIPCHK(hipStreamSynchronize(g_stream));
     return FL_OK;
@@ -444,9 +500,7 @@ static int update_pointers(int sx, int sy, int sz, float *pc)
     if (!D.init) return fail(FL_ERR_STATE, "DRIVER_Update_pointers before DRIVER_Initialize");
     if ((long long)sx * sy * sz != D.n) return fail(FL_ERR_ARG, "grid size differs from the initialised one");
     if (!pc) return FL_OK;
-    HIPCHK(hipStreamSynchronize(g_stream));
-    HIPCHK(hipMemcpy(pc, D.f[1], (size_t)D.n * 4, hipMemcpyDeviceToHost));
-    return FL_OK;
+    return level_xfer(D.lv[D.lc], 0, pc, false, D.sx, D.sy, D.sz, D.px);
 }
 
 extern "C" void DRIVER_Update_pointers(const int sx, const int sy, const int sz, float *pc)
@@ -498,27 +552,24 @@ extern "C" int fl_get_field(int which, float *host)
 {
     if (!D.init) return fail(FL_ERR_STATE, "no initialised grid");
     if (which < 0 || which > 3 || !host) return fail(FL_ERR_ARG, "bad field request %d", which);
-    HIPCHK(hipStreamSynchronize(g_stream));
-    HIPCHK(hipMemcpy(host, D.f[which], (size_t)D.n * 4, hipMemcpyDeviceToHost));
-    return FL_OK;
+    float *lv = (which & 1) ? D.lv[D.lc] : D.lv[1 - D.lc];  // pp pc qp qc
+    return level_xfer(lv, which >> 1, host, false, D.sx, D.sy, D.sz, D.px);
 }
 
 extern "C" int fl_set_field(int which, const float *host)
 {
     if (!D.init) return fail(FL_ERR_STATE, "no initialised grid");
     if (which < 0 || which > 3 || !host) return fail(FL_ERR_ARG, "bad field request %d", which);
-    HIPCHK(hipStreamSynchronize(g_stream));
-    HIPCHK(hipMemcpy(D.f[which], host, (size_t)D.n * 4, hipMemcpyHostToDevice));
-    return FL_OK;
+    float *lv = (which & 1) ? D.lv[D.lc] : D.lv[1 - D.lc];
+    return level_xfer(lv, which >> 1, const_cast<float *>(host), true, D.sx, D.sy, D.sz, D.px);
 }
 
 extern "C" int fl_get_derivative_volume(int k, float *host)
 {
-    if (!D.init || !D.dvol) return fail(FL_ERR_STATE, "no two-pass step has run");
+    if (!D.init || !D.dvx) return fail(FL_ERR_STATE, "no two-pass step has run");
     if (k < 0 || k > 3 || !host) return fail(FL_ERR_ARG, "bad derivative volume %d", k);
-    HIPCHK(hipStreamSynchronize(g_stream));
-    HIPCHK(hipMemcpy(host, D.dvol + k * D.n, (size_t)D.n * 4, hipMemcpyDeviceToHost));
-    return FL_OK;
+    // 0 pDx, 1 pDy, 2 qDx, 3 qDy
+    return level_xfer((k & 1) ? D.dvy : D.dvx, k >> 1, host, false, D.sx, D.sy, D.sz, D.px);
 }
 
 extern "C" int fl_record_traces(int iz, int every, int max_traces)
@@ -589,7 +640,8 @@ extern "C" int fl_run_steps(int it0, int nsteps, int iSource, const float *src, 
     }
     HIPCHK(hipMemcpyAsync(D.src_tab, src, (size_t)nsteps * 4, hipMemcpyHostToDevice, g_stream));
     hipLaunchKernelGGL(set_int, dim3(1), dim3(64), 0, g_stream, D.ctr, 0);
-    hipLaunchKernelGGL(insert_source, dim3(1), dim3(64), 0, g_stream, D.f[1], D.f[3], (long long)iSource, src[0]);
+    const long long psrc = pitched_index(iSource, D.sx, D.sy, D.px);
+    hipLaunchKernelGGL(insert_source, dim3(1), dim3(64), 0, g_stream, D.lv[D.lc], psrc, src[0]);
     const int K = std::min(nsteps, (graph_chunk + 1) / 2 * 2);  // even: levels return to their roles
     auto step_args = [&](int k) {
         StepArgs a = base_args();
@@ -598,7 +650,7 @@ extern "C" int fl_run_steps(int it0, int nsteps, int iSource, const float *src, 
         a.it_next = D.ctr + ((k + 1) & 1);
         a.src_tab = D.src_tab;
         a.src_n = nsteps;
-        a.src_idx = iSource;
+        a.src_idx = psrc;
         return a;
     };
     int done = 0;
@@ -764,11 +816,12 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
     // keep the fields: tuning steps run on them and are undone afterwards
     float *save = nullptr;
     long long dummy = 0;
-    int rc = dev_alloc((void **)&save, (size_t)4 * D.n * 4, &dummy);
+    const size_t lvn = (size_t)2 * D.np;
+    int rc = dev_alloc((void **)&save, 2 * lvn * 4, &dummy);
     if (rc) return rc;
-    float *orig[4] = {D.f[0], D.f[1], D.f[2], D.f[3]};
-    for (int k = 0; k < 4; k++)
-        HIPCHK(hipMemcpyAsync(save + k * D.n, D.f[k], (size_t)D.n * 4, hipMemcpyDeviceToDevice, g_stream));
+    const int lc0 = D.lc;
+    for (int k = 0; k < 2; k++)
+        HIPCHK(hipMemcpyAsync(save + k * lvn, D.lv[k], lvn * 4, hipMemcpyDeviceToDevice, g_stream));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -776,7 +829,7 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
     double best = 1e30;
     Tiles bt = keep;
     for (int cty : {4, 8, 16}) {
-        for (int czt : zt_candidates(std::min(D.sz - 2 * D.bord, max_zt(D.sx, D.sy)))) {
+        for (int czt : zt_candidates(std::min(D.sz - 2 * D.bord, max_zt(D.px, D.sy)))) {
             g_tiles.ty = cty;
             g_tiles.zt = czt;
             StepArgs a = base_args();
@@ -802,10 +855,9 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
         }
         if (rc) break;
     }
-    for (int k = 0; k < 4; k++) {
-        D.f[k] = orig[k];
-        HIPCHK(hipMemcpyAsync(D.f[k], save + k * D.n, (size_t)D.n * 4, hipMemcpyDeviceToDevice, g_stream));
-    }
+    D.lc = lc0;
+    for (int k = 0; k < 2; k++)
+        HIPCHK(hipMemcpyAsync(D.lv[k], save + k * lvn, lvn * 4, hipMemcpyDeviceToDevice, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     (void)hipFree(save);
     (void)hipEventDestroy(e0);

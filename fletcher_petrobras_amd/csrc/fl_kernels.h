@@ -5,12 +5,18 @@
 //   pp <- 2 pc - pp + dt^2 rhsp(pc, qc),   qp <- 2 qc - qp + dt^2 rhsq(pc, qc)
 // over iz,iy,ix in [bord, s-bord) (original/OpenMP/openmp_propagate.c:27-29).
 //
+// Device layout of the wavefields: each time level is one array of (p, q) pairs,
+// cell c = (z*sy + y)*px + x with the row pitch px >= sx a multiple of 16 cells
+// (128-byte rows), p at 2c and q at 2c+1.  A step reads the current level (stencil) and
+// the previous one (own cell) and overwrites the previous one; the roles then swap.
+// The coefficients are records of one cell each, at the same pitched cell index.
+//
 // Kernels here (the z-marching step kernel is in fl_step.h):
 //   direct_dercross  the reference's direct 64-point DerCross form
 //       (original/derivatives.h:59-68), one thread per (x,y) column.
 //   der1_pass        pass 1 of the two-pass form (der1der1/OpenMP/openmp_propagate.c:24-76).
 //   insert_source    p[i] += s, q[i] += s (original/OpenMP/openmp_insertsource.c:7-12).
-//   gather_plane     receiver plane copy.
+//   field_pack / field_unpack   one field between the reference layout and a level.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -47,21 +53,20 @@ enum CoefMode : int {
 };
 
 struct StepArgs {
-    float *pp;
-    const float *pc;
-    float *qp;
-    const float *qc;
+    float *prev;          // level read at the own cell and overwritten (pp, qp)
+    const float *cur;     // level read with the stencil (pc, qc)
     const void *coef;     // coefficient records: per cell its values, consecutive
-    const float *dvol;    // TWOPASS: pDx, pDy, qDx, qDy volumes, n elements apart
+    const float *dvx, *dvy;  // TWOPASS: (pDx, qDx) and (pDy, qDy) pair volumes
     float dxxinv, dyyinv, dzzinv, dxinv, dyinv, dzinv, dt;
-    int sx, sy, sz, bord;
+    int sx, sy, sz, bord, px;
     int ntx, nty, zt, nblk;  // tiles along x, y; z chunk depth; logical blocks
     // step index it: it_host, or it_base + 1 + *it_dev inside a captured run, where
     // the kernel also publishes *it_next = *it_dev + 1 for the following step
     int it_host, it_base;
     const int *it_dev;
     int *it_next;
-    // source of the next step added to this step's output (captured runs)
+    // source of the next step added to this step's output (captured runs); src_idx is
+    // the pitched cell index
     const float *src_tab;
     int src_n;
     long long src_idx;
@@ -226,18 +231,17 @@ __device__ __forceinline__ float der1_g(const float *__restrict__ p, long long i
 }
 
 struct DirectArgs {
-    float *pp;
-    const float *pc;
-    float *qp;
-    const float *qc;
+    float *prev;
+    const float *cur;
     const float *coef;    // FP32 records of 10 per cell
     float dxxinv, dyyinv, dzzinv, dxyinv, dxzinv, dyzinv, dt;
-    int sx, sy, sz, bord, zt;
+    int sx, sy, sz, bord, px, zt;
     float *trace;
     int rec_iz;
 };
 
-// grid: (ceil(nx/64), ceil(ny/4), nzchunks); block (64, 4)
+// grid: (ceil(nx/64), ceil(ny/4), nzchunks); block (64, 4).  Fields are (p, q) pairs:
+// element strides 2 (x), 2 px (y), 2 px sy (z); p at even, q at odd elements.
 __global__ __launch_bounds__(256) void direct_dercross(const DirectArgs a)
 {
     const int x = a.bord + blockIdx.x * 64 + threadIdx.x;
@@ -245,74 +249,85 @@ __global__ __launch_bounds__(256) void direct_dercross(const DirectArgs a)
     if (x >= a.sx - a.bord || y >= a.sy - a.bord) return;
     const int zlo = a.bord + blockIdx.z * a.zt;
     const int zhi = min(zlo + a.zt, a.sz - a.bord);
-    const long long sX = 1, sY = a.sx, sZ = (long long)a.sx * a.sy;
+    const long long sX = 2, sY = 2ll * a.px, sZ = 2ll * a.px * a.sy;
+    const float *pc = a.cur, *qc = a.cur + 1;
     for (int z = zlo; z < zhi; z++) {
-        const long long i = z * sZ + (long long)y * a.sx + x;
-        const float pxx = der2_g(a.pc, i, sX, a.dxxinv);
-        const float pyy = der2_g(a.pc, i, sY, a.dyyinv);
-        const float pzz = der2_g(a.pc, i, sZ, a.dzzinv);
-        const float pxy = dercross_g(a.pc, i, sX, sY, a.dxyinv);
-        const float pyz = dercross_g(a.pc, i, sY, sZ, a.dyzinv);
-        const float pxz = dercross_g(a.pc, i, sX, sZ, a.dxzinv);
-        const float qxx = der2_g(a.qc, i, sX, a.dxxinv);
-        const float qyy = der2_g(a.qc, i, sY, a.dyyinv);
-        const float qzz = der2_g(a.qc, i, sZ, a.dzzinv);
-        const float qxy = dercross_g(a.qc, i, sX, sY, a.dxyinv);
-        const float qyz = dercross_g(a.qc, i, sY, sZ, a.dyzinv);
-        const float qxz = dercross_g(a.qc, i, sX, sZ, a.dxzinv);
+        const long long c = ((long long)z * a.sy + y) * a.px + x, i = 2 * c;
+        const float pxx = der2_g(pc, i, sX, a.dxxinv);
+        const float pyy = der2_g(pc, i, sY, a.dyyinv);
+        const float pzz = der2_g(pc, i, sZ, a.dzzinv);
+        const float pxy = dercross_g(pc, i, sX, sY, a.dxyinv);
+        const float pyz = dercross_g(pc, i, sY, sZ, a.dyzinv);
+        const float pxz = dercross_g(pc, i, sX, sZ, a.dxzinv);
+        const float qxx = der2_g(qc, i, sX, a.dxxinv);
+        const float qyy = der2_g(qc, i, sY, a.dyyinv);
+        const float qzz = der2_g(qc, i, sZ, a.dzzinv);
+        const float qxy = dercross_g(qc, i, sX, sY, a.dxyinv);
+        const float qyz = dercross_g(qc, i, sY, sZ, a.dyzinv);
+        const float qxz = dercross_g(qc, i, sX, sZ, a.dxzinv);
         Coef k;
 #pragma unroll
-        for (int j = 0; j < 10; j++) k.c[j] = a.coef[i * 10 + j];
-        float ppv = a.pp[i], qpv = a.qp[i];
+        for (int j = 0; j < 10; j++) k.c[j] = a.coef[c * 10 + j];
+        float ppv = a.prev[i], qpv = a.prev[i + 1];
         combine<CM_F32>(k, a.dt, pxx, pyy, pzz, pxy, pyz, pxz, qxx, qyy, qzz, qxy, qyz, qxz,
-                        a.pc[i], a.qc[i], ppv, qpv);
+                        pc[i], qc[i], ppv, qpv);
         if (a.trace && z == a.rec_iz)
             a.trace[(long long)(y - a.bord) * (a.sx - 2 * a.bord) + (x - a.bord)] = ppv;
-        a.pp[i] = ppv;
-        a.qp[i] = qpv;
+        a.prev[i] = ppv;
+        a.prev[i + 1] = qpv;
     }
 }
 
 // ---------------------------------------------------------------------------------
 // Pass 1 of the two-pass form: Dx of pc,qc over ix in [bord, sx-bord), all iy;
 // Dy over iy in [bord, sy-bord), ix in [bord, sx); every plane iz in [0, sz).
+// dvx, dvy: (pDx, qDx), (pDy, qDy) pairs at the pitched cell index.
 // grid: (ceil(sx/64), ceil(sy/4), sz); block (64, 4)
-__global__ __launch_bounds__(256) void der1_pass(const float *__restrict__ pc,
-                                                 const float *__restrict__ qc, float *dvol, int sx,
-                                                 int sy, int sz, int bord, float dxinv, float dyinv)
+__global__ __launch_bounds__(256) void der1_pass(const float *__restrict__ cur, float *dvx, float *dvy,
+                                                 int sx, int sy, int px, int bord, float dxinv,
+                                                 float dyinv)
 {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * 4 + threadIdx.y;
     const int z = blockIdx.z;
     if (x >= sx || y >= sy) return;
-    const long long n = (long long)sx * sy * sz;
-    const long long i = ((long long)z * sy + y) * sx + x;
+    const long long i = 2 * (((long long)z * sy + y) * px + x);
     if (x >= bord && x < sx - bord) {
-        dvol[i] = der1_g(pc, i, 1, dxinv);
-        dvol[2 * n + i] = der1_g(qc, i, 1, dxinv);
+        dvx[i] = der1_g(cur, i, 2, dxinv);
+        dvx[i + 1] = der1_g(cur + 1, i, 2, dxinv);
     }
     if (y >= bord && y < sy - bord && x >= bord) {
-        dvol[n + i] = der1_g(pc, i, sx, dyinv);
-        dvol[3 * n + i] = der1_g(qc, i, sx, dyinv);
+        dvy[i] = der1_g(cur, i, 2ll * px, dyinv);
+        dvy[i + 1] = der1_g(cur + 1, i, 2ll * px, dyinv);
     }
 }
 
-__global__ void insert_source(float *p, float *q, long long i, float s)
+// Source insertion at pitched cell c of a level: p += s, q += s.
+__global__ void insert_source(float *lv, long long c, float s)
 {
     if (threadIdx.x == 0) {
-        p[i] += s;
-        q[i] += s;
+        lv[2 * c] += s;
+        lv[2 * c + 1] += s;
     }
 }
 
-// Receiver-plane gather: pc on plane iz over [bord, sx-bord) x [bord, sy-bord).
-__global__ void gather_plane(const float *__restrict__ pc, float *out, int sx, int sy, int bord, int iz)
+// One field (fld 0 p, 1 q) of a level <-> a volume in the reference layout.
+// grid: (ceil(sx/256), sy, sz); block 256
+__global__ __launch_bounds__(256) void field_pack(const float *__restrict__ lin, float *lv, int fld,
+                                                  int sx, int sy, int px)
 {
-    const int nxr = sx - 2 * bord, nyr = sy - 2 * bord;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nxr * nyr) return;
-    const int iy = e / nxr, ix = e - iy * nxr;
-    out[e] = pc[((long long)iz * sy + iy + bord) * sx + ix + bord];
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= sx) return;
+    const long long row = (long long)blockIdx.z * sy + blockIdx.y;
+    lv[2 * (row * px + x) + fld] = lin[row * sx + x];
+}
+__global__ __launch_bounds__(256) void field_unpack(const float *__restrict__ lv, float *lin, int fld,
+                                                    int sx, int sy, int px)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= sx) return;
+    const long long row = (long long)blockIdx.z * sy + blockIdx.y;
+    lin[row * sx + x] = lv[2 * (row * px + x) + fld];
 }
 
 }  // namespace fl

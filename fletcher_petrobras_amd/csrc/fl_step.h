@@ -2,8 +2,8 @@
 //
 // Work decomposition.  A workgroup of TY waves owns a 32 x TY tile of the xy plane and a
 // chunk [zlo, zhi) of z; it streams the input planes f = zlo-4 .. zhi+3 of pc, qc through
-// LDS once.  A wave covers 32 columns of one tile row for BOTH fields: lanes 0-31 carry p,
-// lanes 32-63 carry q of the same 32 cells, so each lane keeps only its own field's state.
+// LDS once.  A wave covers 32 columns of one tile row for BOTH fields: lane 2c carries p
+// and lane 2c+1 q of column c, so each lane keeps only its own field's state.
 //
 // On chip, per lane (own field, own column):
 //   registers  9-deep z queues of c = pc|qc, Dx and Dy over planes f-8 .. f (Dzz, Dxz, Dyz
@@ -18,13 +18,15 @@
 // With the rings every plane needs a single barrier: nothing written in a step is read
 // in it, and nothing read in it is overwritten before the next barrier.
 //
-// Memory.  Every stream goes through a buffer resource over the chunk's span with a
+// Memory.  The levels hold (p, q) pairs per cell (pitched rows), so a wave's 32 cells x 2
+// fields are 64 consecutive floats; the coefficients are one record per cell.  Every
+// stream goes through a buffer resource over the chunk's span with a
 // wave-uniform plane offset (soffset): one SGPR add per plane, no 64-bit address math.
 // Lanes outside the grid carry an offset past the range, so their loads return 0 and
 // their stores are dropped by the hardware (no masks, no clamps).
 //
 // The combination and leapfrog follow original/sample.h:46-86 in its association order;
-// p and q exchange h1q / h2p with one cross-half lane swap (v_permlane32_swap).
+// p and q exchange h1q / h2p with one swap between neighbouring lanes (DPP).
 //
 // TWOPASS takes Dx, Dy from the four first-derivative volumes of der1_pass
 // (der1der1/OpenMP/openmp_propagate.c:24-76) instead of forming them on chip.
@@ -141,9 +143,9 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, Coef &k)
 
 template <int TY>
 struct StepLds {
-    static constexpr int HX = 40, HY = TY + 8, HT = HX * HY, XT = 32 * HY;
-    float t[6][2 * HT];  // staged pc | qc planes, tile + halo, p then q
-    float x[5][2 * XT];  // Dx of pc | qc, tile columns x (tile rows + y halo)
+    static constexpr int HX = 40, HY = TY + 8;
+    float t[6][HY * HX * 2];  // staged levels: (row, column, field) of the tile + halo
+    float x[5][HY * 32 * 2];  // Dx: (row, tile column, field), tile rows + y halo
 };
 
 template <int V>
@@ -158,102 +160,97 @@ __device__ __forceinline__ void unroll_phases(F &&fn, std::integer_sequence<int,
     (fn(ic<P>{}), ...);
 }
 
-// One tile of one z chunk.  Output planes z in [zlo, zhi) of pp/qp are written once.
+// One tile of one z chunk.  Output planes z in [zlo, zhi) of the previous level are
+// overwritten once.
 template <int TY, int CM, bool TWOPASS>
 __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl, int tix, int tiy, int zc,
                                           StepLds<TY> &s)
 {
-    constexpr int TX = 32, HX = 40, HY = TY + 8, HT = HX * HY, XT = TX * HY, NT = 64 * TY;
-    constexpr int HTP = (HT + 63) / 64 * 64;  // field stride of the staging index space
-    constexpr int NLD = (2 * HTP + NT - 1) / NT;
+    constexpr int TX = 32, HX = 40, HY = TY + 8, NT = 64 * TY;
+    constexpr int RW = 2 * HX;                 // floats per staged row (pairs)
+    constexpr int NS = RW * HY;                // floats per staged plane (a multiple of 64)
+    constexpr int NLD = (NS + NT - 1) / NT;    // staging loads per lane
+    constexpr int XW2 = 2 * TX;                // floats per Dx row
     constexpr int REC = (CM == CM_F32) ? 40 : (CM == CM_F16R ? 12 : 20);  // record bytes
-    constexpr int NH = (8 + TY - 1) / TY;     // y-halo rows of Dx per wave (at most)
+    constexpr int NH = (8 + TY - 1) / TY;      // y-halo rows of Dx per wave (at most)
+    static_assert(NS % 64 == 0, "staged plane must be whole waves");
     const int x0 = tix * TX, y0 = a.bord + tiy * TY;
     const int zlo = a.bord + zc * a.zt;
     const int zhi = min(zlo + a.zt, a.sz - a.bord);
     const int nz = zhi - zlo, f0 = zlo - 4;
     const int lane = threadIdx.x;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // tile row (wave-uniform)
-    const int fld = lane >> 5, cx = lane & 31;
+    // lane = 2 * column + field: a half wave reads 32 consecutive floats (16 cells x 2
+    // fields) from global memory and LDS alike, free of LDS bank conflicts
+    const int fld = lane & 1, cx = lane >> 1;
     const int x = x0 + cx, y = y0 + wv;
     const bool active = (x >= a.bord) && (x < a.sx - a.bord) && (y < a.sy - a.bord);
-    const uint32_t sxy = (uint32_t)a.sx * (uint32_t)a.sy;
-    const uint32_t pb4 = sxy * 4u, pbr = sxy * (uint32_t)REC;
-    const uint32_t col = (uint32_t)(y * a.sx + x);
-    // clamped into the grid (TWOPASS global loads of lanes outside the update range)
-    const uint32_t colr = (uint32_t)(min(y, a.sy - 1) * a.sx + min(x, a.sx - 1));
-    const uint32_t vo4 = active ? col * 4u : FL_OOB;
+    const uint32_t pxy = (uint32_t)a.px * (uint32_t)a.sy;  // cells per pitched plane
+    const uint32_t pb8 = pxy * 8u, pbr = pxy * (uint32_t)REC;
+    const uint32_t col = (uint32_t)(y * a.px + x);
+    const uint32_t vo8 = active ? col * 8u + fld * 4u : FL_OOB;  // own (cell, field) float
     const uint32_t vor = active ? col * (uint32_t)REC : FL_OOB;
-    const uint32_t vop = fld ? FL_OOB : vo4, voq = fld ? vo4 : FL_OOB;  // own field only
 
-    const long long zin = (long long)f0 * sxy, zout = (long long)zlo * sxy;
-    const __amdgpu_buffer_rsrc_t r_pc = span_rsrc(a.pc + zin, (uint32_t)(nz + 8) * pb4);
-    const __amdgpu_buffer_rsrc_t r_qc = span_rsrc(a.qc + zin, (uint32_t)(nz + 8) * pb4);
-    const __amdgpu_buffer_rsrc_t r_pp = span_rsrc(a.pp + zout, (uint32_t)nz * pb4);
-    const __amdgpu_buffer_rsrc_t r_qp = span_rsrc(a.qp + zout, (uint32_t)nz * pb4);
+    const long long zin = (long long)f0 * pxy, zout = (long long)zlo * pxy;
+    const __amdgpu_buffer_rsrc_t r_cur = span_rsrc(a.cur + 2 * zin, (uint32_t)(nz + 8) * pb8);
+    const __amdgpu_buffer_rsrc_t r_prev = span_rsrc(a.prev + 2 * zout, (uint32_t)nz * pb8);
     const char *cpl = (const char *)a.coef + zout * REC;  // records of the next output plane
 
     // source cell of this step (added to both fields of the output) and receiver plane
     int src_z = -1;
     uint32_t src_col = 0;
     if (ctl.do_src && a.src_idx >= 0) {
-        src_z = (int)(a.src_idx / (long long)sxy);
-        src_col = (uint32_t)(a.src_idx - (long long)src_z * sxy);
+        src_z = (int)(a.src_idx / (long long)pxy);
+        src_col = (uint32_t)(a.src_idx - (long long)src_z * pxy);
     }
     const int rec_z = ctl.trace ? a.rec_iz : -1;
 
-    // staging: element e' = wave*64 + k*NT + lane of [0, 2*HTP); p for e' < HTP, q above;
-    // the field of a (wave, k) pair is uniform
+    // staging: float e of a staged plane = row*RW + 2*column + field, the same index in
+    // LDS; wave w loads e = w*64 + k*NT + lane (whole waves, contiguous per wave)
     uint32_t hoff[NLD];
-    int hs[NLD];  // LDS element of lane 0 (wave-uniform); lane l stores element hs + l
-    bool hq[NLD], hk[NLD];
+    bool hk[NLD];
 #pragma unroll
     for (int k = 0; k < NLD; k++) {
         const int ew = wv * 64 + k * NT;
-        hk[k] = ew < 2 * HTP;
-        hq[k] = ew >= HTP;
-        const int e0 = ew - (hq[k] ? HTP : 0);  // element within the field, lane 0
-        const int ee = e0 + lane;
-        const bool inb = hk[k] && (HT == HTP || ee < HT);
-        const int r = ee / HX, c = ee - r * HX;
-        const int gy = y0 - 4 + r, gx = x0 - 4 + c;
-        const bool ok = inb && gx >= 0 && gx < a.sx && gy < a.sy;
-        hoff[k] = ok ? (uint32_t)(gy * a.sx + gx) * 4u : FL_OOB;
-        hs[k] = (hq[k] ? HT : 0) + e0;
+        hk[k] = ew < NS;
+        const int e = ew + lane;
+        const int r = e / RW, w = e - r * RW;
+        const int gy = y0 - 4 + r, gx = x0 - 4 + (w >> 1);
+        const bool ok = hk[k] && gx >= 0 && gx < a.sx && gy < a.sy;
+        hoff[k] = ok ? ((uint32_t)(gy * a.px + gx) * 2u + (uint32_t)(w & 1)) * 4u : FL_OOB;
     }
     float pre[NLD];
     auto fetch = [&](int i) {  // raw loads of input plane f0 + i
-        const uint32_t so = (uint32_t)i * pb4;
+        const uint32_t so = (uint32_t)i * pb8;
 #pragma unroll
         for (int k = 0; k < NLD; k++)
-            if (hk[k]) pre[k] = __builtin_bit_cast(float, bload32(hq[k] ? r_qc : r_pc, hoff[k], so));
+            if (hk[k]) pre[k] = __builtin_bit_cast(float, bload32(r_cur, hoff[k], so));
     };
     auto stage = [&](float *T) {
 #pragma unroll
         for (int k = 0; k < NLD; k++)
-            if (hk[k] && (HT == HTP || hs[k] + lane < (hq[k] ? 2 * HT : HT))) T[hs[k] + lane] = pre[k];
+            if (hk[k]) T[wv * 64 + k * NT + lane] = pre[k];
     };
 
-    // lane offsets in the LDS planes
-    // (window corners, so that every stencil read is a non-negative immediate offset)
-    const int ow = fld * HT + wv * HX + cx;  // 9 x 9 window of the own cell in a staged plane
-    const int oxw = fld * XT + wv * TX + cx; // 9-row column of the own cell in a Dx plane
-    constexpr int CW = 4 * HX + 4, CX = 4 * TX;  // the own cell within them
-    int ohr[NH], ohx[NH];                              // y-halo rows of Dx done by this wave
+    // lane offsets in the LDS planes (window corners: every stencil read is a
+    // non-negative immediate offset)
+    const int ow = wv * RW + lane;   // 9 x 9 window of the own (cell, field)
+    const int oxw = wv * XW2 + lane; // 9-row column of the own (cell, field) in Dx
+    constexpr int CW = 4 * RW + 8, CX = 4 * XW2;  // the own cell within them
+    int ohr[NH], ohx[NH];                     // y-halo rows of Dx done by this wave
     bool hrow[NH];
 #pragma unroll
     for (int m = 0; m < NH; m++) {
         const int j = wv + m * TY;  // halo row index 0..7 (rows 0-3 and TY+4..TY+7)
         hrow[m] = j < 8;
         const int r = j < 4 ? j : TY + j;
-        ohr[m] = fld * HT + r * HX + cx;
-        ohx[m] = fld * XT + r * TX + cx;
+        ohr[m] = r * RW + lane;
+        ohx[m] = r * XW2 + lane;
     }
 
-    // TWOPASS: first-derivative volumes of the own field
-    const long long nvol = (long long)sxy * a.sz;
-    const float *dvx = TWOPASS ? a.dvol + (fld ? 2 * nvol : 0) : nullptr;
-    const float *dvy = TWOPASS ? a.dvol + (fld ? 3 * nvol : nvol) : nullptr;
+    // TWOPASS: first-derivative pair volumes of pass 1, over the chunk's input planes
+    const __amdgpu_buffer_rsrc_t r_dx = span_rsrc(TWOPASS ? a.dvx + 2 * zin : a.cur, (uint32_t)(nz + 8) * pb8);
+    const __amdgpu_buffer_rsrc_t r_dy = span_rsrc(TWOPASS ? a.dvy + 2 * zin : a.cur, (uint32_t)(nz + 8) * pb8);
 
     // own field c, Dx, Dy of planes f-8..f: ring of 10 slots (plane j in slot j % 10 in
     // the statically addressed steps; 10 keeps the step parity of every slot fixed)
@@ -262,7 +259,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     for (int k = 0; k < 10; k++) qc_[k] = qx_[k] = qy_[k] = 0.0f;
     // output-plane streams loaded one step ahead (set = parity of the consuming step)
     CoefRec<CM> kp[2];
-    float ppp[2] = {0.0f, 0.0f}, qpp[2] = {0.0f, 0.0f};
+    float ppv[2] = {0.0f, 0.0f};
 
     // prologue: plane f0 into ring slot 0, loads of plane f0+1 in flight
     fetch(0);
@@ -292,42 +289,42 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         float *const Tz = s.t[r6 >= 4 ? r6 - 4 : r6 + 2];
         float *const Xf = s.x[r5];
         float *const Xz = s.x[r5n];
-        const uint32_t so_o = (uint32_t)(i - 8) * pb4;
+        const uint32_t so_o = (uint32_t)(i - 8) * pb8;
 
         // streams of the output plane, kept raw until used
         CoefRec<CM> kr;
-        float vpp = 0.0f, vqp = 0.0f;
+        float vprev = 0.0f;
         if constexpr (OUT && (PF & 1)) {
             kr = kp[SET];
-            vpp = ppp[SET];
-            vqp = qpp[SET];
+            vprev = ppv[SET];
         } else if constexpr (OUT) {
             load_rec<CM>(span_rsrc(cpl, pbr), vor, kr);
             cpl += pbr;
-            vpp = __builtin_bit_cast(float, bload32(r_pp, vop, so_o));
-            vqp = __builtin_bit_cast(float, bload32(r_qp, voq, so_o));
+            vprev = __builtin_bit_cast(float, bload32(r_prev, vo8, so_o));
         }
         const bool xw = (XW == 1) || (XW == 2 && i < nz + 4);
         float nx, ny, gxh[NH];
         if constexpr (TWOPASS) {
-            const long long fo = (long long)f * sxy;
-            nx = ld_f32(dvx + fo, colr * 4u);
-            ny = ld_f32(dvy + fo, colr * 4u);
+            const uint32_t so = (uint32_t)i * pb8;
+            nx = __builtin_bit_cast(float, bload32(r_dx, vo8, so));
+            ny = __builtin_bit_cast(float, bload32(r_dy, vo8, so));
             if (xw) {
 #pragma unroll
                 for (int m = 0; m < NH; m++)
                     if (hrow[m]) {
-                        const int r = (wv + m * TY) < 4 ? (wv + m * TY) : TY + wv + m * TY;
-                        const int gy = y0 - 4 + r;
-                        gxh[m] = (x < a.sx && gy < a.sy) ? ld_f32(dvx + fo, (uint32_t)(gy * a.sx + x) * 4u) : 0.0f;
+                        const int j = wv + m * TY;
+                        const int gy = y0 - 4 + (j < 4 ? j : TY + j);
+                        const uint32_t o = (x < a.sx && gy < a.sy)
+                                               ? ((uint32_t)(gy * a.px + x) * 2u + fld) * 4u : FL_OOB;
+                        gxh[m] = __builtin_bit_cast(float, bload32(r_dx, o, so));
                     }
             }
         }
         const float *cp = Tf + ow + CW;
         float nc = cp[0];
         if constexpr (!TWOPASS) {
-            nx = der1_lds(cp, 1, a.dxinv);
-            ny = der1_lds(cp, HX, a.dyinv);
+            nx = der1_lds(cp, 2, a.dxinv);
+            ny = der1_lds(cp, RW, a.dyinv);
         }
         if (xw) {
             Xf[oxw + CX] = nx;
@@ -335,7 +332,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             for (int m = 0; m < NH; m++)
                 if (hrow[m]) {
                     if constexpr (TWOPASS) Xf[ohx[m]] = gxh[m];
-                    else Xf[ohx[m]] = der1_lds(Tf + ohr[m] + 4, 1, a.dxinv);
+                    else Xf[ohx[m]] = der1_lds(Tf + ohr[m] + 8, 2, a.dxinv);
                 }
         }
         // formed here, in this step: keeps the compiler from carrying the raw LDS values to
@@ -351,9 +348,9 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         float dxx = 0.f, dyy = 0.f, dxy = 0.f;
         if constexpr (OUT) {
             const float *zp = Tz + ow + CW;
-            dxx = der2_lds(zp, 1, a.dxxinv);
-            dyy = der2_lds(zp, HX, a.dyyinv);
-            dxy = der1_lds(Xz + oxw + CX, TX, a.dyinv);
+            dxx = der2_lds(zp, 2, a.dxxinv);
+            dyy = der2_lds(zp, RW, a.dyyinv);
+            dxy = der1_lds(Xz + oxw + CX, XW2, a.dyinv);
         }
         // next plane into the ring, the one after it in flight
         if (i + 1 <= nz + 7) stage(s.t[r6n]);
@@ -381,10 +378,10 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             h1 = __builtin_fmaf(kc.c[5], dxz, h1);
             h1 = __builtin_fmaf(kc.c[4], dyz, h1);
             const float h2 = dxx + dyy + dzz - h1;
-            // lanes 0-31 receive h1q, lanes 32-63 receive h2p
-            const auto sw = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, h1),
-                                                             __builtin_bit_cast(uint32_t, h2), false, false);
-            const float other = __builtin_bit_cast(float, fld ? sw[0] : sw[1]);
+            // p lanes receive h1q, q lanes h2p: swap within lane pairs (DPP quad_perm 1,0,3,2)
+            const float send = fld ? h1 : h2;
+            const float other = __builtin_bit_cast(
+                float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, send), 0xB1, 0xF, 0xF, false));
             // rhsp = v2px h2p + v2pz h1q + v2sz (h1p - h1q)
             // rhsq = v2pn h2p + v2pz h1q - v2sz (h2p - h2q) = ... + v2sz (h2q - h2p)
             // with h2p, h1q selected per half (original/sample.h:75-81, same roundings)
@@ -393,8 +390,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             rhs = __builtin_fmaf(kc.c[7], h1q, rhs);
             rhs = __builtin_fmaf(kc.c[8], (fld ? h2 : h1) - other, rhs);
             const float cur = qz[4];
-            const float prev = fld ? vqp : vpp;
-            const float base = 2.0f * cur - prev;
+            const float base = 2.0f * cur - vprev;
             float nw;
             if constexpr (CM == CM_F32) nw = __builtin_fmaf(rhs * a.dt, a.dt, base);
             else nw = base + rhs;
@@ -405,16 +401,13 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             if (z == src_z) {
                 if (col == src_col && active) nw += ctl.src_add;
             }
-            bstore32(r_pp, vop, so_o, nw);
-            bstore32(r_qp, voq, so_o, nw);
+            bstore32(r_prev, vo8, so_o, nw);
         }
         if constexpr ((PF & 2) != 0 || (PF & 4) != 0) {
             if ((PF & 2) || pf_next) {
-                const uint32_t so_n = (uint32_t)(i - 7) * pb4;
                 load_rec<CM>(span_rsrc(cpl, pbr), vor, kp[SET ^ 1]);
                 cpl += pbr;
-                ppp[SET ^ 1] = __builtin_bit_cast(float, bload32(r_pp, vop, so_n));
-                qpp[SET ^ 1] = __builtin_bit_cast(float, bload32(r_qp, voq, so_n));
+                ppv[SET ^ 1] = __builtin_bit_cast(float, bload32(r_prev, vo8, (uint32_t)(i - 7) * pb8));
             }
         }
         if constexpr (GEN) {
@@ -458,18 +451,19 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     for (; i <= nz + 7; i++) step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, i, false);
 }
 
-// Minimum waves per SIMD requested from the register allocator (0: compiler default).
-#ifndef FL_STEP_WAVES
-#define FL_STEP_WAVES 4
-#endif
-#if FL_STEP_WAVES > 0
-#define FL_STEP_ATTR __attribute__((amdgpu_waves_per_eu(FL_STEP_WAVES)))
-#else
-#define FL_STEP_ATTR
-#endif
+// Waves per SIMD the register allocator is asked for.  With 8-row tiles the LDS rings
+// (51 KB per workgroup) allow three workgroups per CU: 6 waves per SIMD, <= 80 VGPRs,
+// which the 16-bit coefficient forms meet; FP32 records keep two (4 waves, <= 128).
+// 4- and 16-row tiles are held to 4 waves per SIMD by their LDS.
+template <int TY, int CM>
+constexpr int step_waves()
+{
+    return (TY == 8 && CM != CM_F32) ? 6 : 4;
+}
 
 template <int TY, int CM, bool TWOPASS>
-__global__ __launch_bounds__(64 * TY) FL_STEP_ATTR void tti_step(const StepArgs a)
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(step_waves<TY, CM>())))
+void tti_step(const StepArgs a)
 {
     __shared__ StepLds<TY> s;
     if (a.it_next && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0)
@@ -484,11 +478,11 @@ __global__ __launch_bounds__(64 * TY) FL_STEP_ATTR void tti_step(const StepArgs 
 
 // Shot of a batched run (device copy of the host descriptor).
 struct ShotDev {
-    float *p[2], *q[2];
+    float *lv[2];       // levels of (p, q) pairs; lv[parity] is the previous one
     const void *coef;   // coefficient records
     float *trace;
-    long long src_idx;
-    int sx, sy, sz, rec_iz;
+    long long src_idx;  // pitched cell index
+    int sx, sy, sz, px, rec_iz;
     int ntx, nty, item0, nitems;  // tiles along x, y; first work item; item count
 };
 
@@ -510,13 +504,13 @@ __global__ void batch_insert(const ShotDev *shots, int nshots, int parity, float
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nshots) return;
     const ShotDev &sh = shots[i];
-    sh.p[1 - parity][sh.src_idx] += s;
-    sh.q[1 - parity][sh.src_idx] += s;
+    sh.lv[1 - parity][2 * sh.src_idx] += s;
+    sh.lv[1 - parity][2 * sh.src_idx + 1] += s;
 }
 
 // Batched shots: a persistent grid walks the (shot, tile) work list.
 template <int TY, int CM>
-__global__ __launch_bounds__(64 * TY) FL_STEP_ATTR void tti_step_batch(const BatchArgs b)
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) void tti_step_batch(const BatchArgs b)
 {
     __shared__ StepLds<TY> s;
     const int k = *b.it_dev;
@@ -530,15 +524,13 @@ __global__ __launch_bounds__(64 * TY) FL_STEP_ATTR void tti_step_batch(const Bat
         }
         const ShotDev &sh = b.shots[lo];
         StepArgs a;
-        a.pp = sh.p[b.parity];
-        a.pc = sh.p[1 - b.parity];
-        a.qp = sh.q[b.parity];
-        a.qc = sh.q[1 - b.parity];
+        a.prev = sh.lv[b.parity];
+        a.cur = sh.lv[1 - b.parity];
         a.coef = sh.coef;
-        a.dvol = nullptr;
+        a.dvx = a.dvy = nullptr;
         a.dxxinv = b.dxxinv; a.dyyinv = b.dyyinv; a.dzzinv = b.dzzinv;
         a.dxinv = b.dxinv; a.dyinv = b.dyinv; a.dzinv = b.dzinv; a.dt = b.dt;
-        a.sx = sh.sx; a.sy = sh.sy; a.sz = sh.sz; a.bord = b.bord;
+        a.sx = sh.sx; a.sy = sh.sy; a.sz = sh.sz; a.bord = b.bord; a.px = sh.px;
         a.ntx = sh.ntx; a.nty = sh.nty; a.zt = b.zt; a.nblk = sh.nitems;
         a.rec_iz = sh.rec_iz;
         a.src_idx = sh.src_idx;

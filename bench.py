@@ -25,7 +25,7 @@ extras (skipped with --quick), each timed over its whole run with inputs residen
             receivers on plane z=28 every 4 steps, 50-step graph chunks
   batched   32 shots on uneven (128..256)^2 x 320 apertures cut from one seeded
             512x512x320 model (absorb 16), 800 steps in 50-step graphs, autotuned
-            tiles, FP32 and BF16 coefficients; receivers every 4 steps
+            tiles, FP32, BF16 and FP16-recompute coefficients; receivers every 4 steps
 """
 import argparse
 import json
@@ -233,8 +233,8 @@ def main():
             r = graph_run(F, np, model, shape, dt, isrc, "fused", prec, args.steps, rec_iz, "auto")
             r["vs_fp32_dropin"] = round(r["value"] / value, 3)
             extras["mixed"][prec] = r
-        bt = batched(F, np, ("fp32", "bf16"), args.batch_steps, args.shots)
-        for prec in ("fp32", "bf16"):
+        bt = batched(F, np, ("fp32", "bf16", "fp16r"), args.batch_steps, args.shots)
+        for prec in ("fp32", "bf16", "fp16r"):
             bt[prec]["vs_fp32_dropin"] = round(bt[prec]["value"] / value, 3)
         extras["batched"] = bt
 

@@ -508,46 +508,54 @@ __global__ void batch_insert(const ShotDev *shots, int nshots, int parity, float
     sh.lv[1 - parity][2 * sh.src_idx + 1] += s;
 }
 
-// Batched shots: a persistent grid walks the (shot, tile) work list.
+// Batched shots: one workgroup per (shot, tile) item of the work list.
+// The batched kernel keeps a shot descriptor besides the tile state: 6 waves per SIMD
+// leave its FP32 and stored 16-bit forms spilling, the recomputed form does not.
 template <int TY, int CM>
-__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) void tti_step_batch(const BatchArgs b)
+constexpr int batch_waves()
+{
+    return (TY == 8 && CM == CM_F16R) ? 6 : 4;
+}
+template <int TY, int CM>
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(batch_waves<TY, CM>())))
+void tti_step_batch(const BatchArgs b)
 {
     __shared__ StepLds<TY> s;
     const int k = *b.it_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *b.it_next = k + 1;
     const int it = b.it_base + 1 + k;
-    for (int t = xcd_remap(blockIdx.x, gridDim.x); t < b.nitems; t += gridDim.x) {
-        int lo = 0, hi = b.nshots - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (b.shots[mid].item0 <= t) lo = mid; else hi = mid - 1;
-        }
-        const ShotDev &sh = b.shots[lo];
-        StepArgs a;
-        a.prev = sh.lv[b.parity];
-        a.cur = sh.lv[1 - b.parity];
-        a.coef = sh.coef;
-        a.dvx = a.dvy = nullptr;
-        a.dxxinv = b.dxxinv; a.dyyinv = b.dyyinv; a.dzzinv = b.dzzinv;
-        a.dxinv = b.dxinv; a.dyinv = b.dyinv; a.dzinv = b.dzinv; a.dt = b.dt;
-        a.sx = sh.sx; a.sy = sh.sy; a.sz = sh.sz; a.bord = b.bord; a.px = sh.px;
-        a.ntx = sh.ntx; a.nty = sh.nty; a.zt = b.zt; a.nblk = sh.nitems;
-        a.rec_iz = sh.rec_iz;
-        a.src_idx = sh.src_idx;
-        StepCtl ctl;
-        ctl.do_src = b.src_tab && (k + 1) < b.src_n;
-        ctl.src_add = ctl.do_src ? b.src_tab[k + 1] : 0.0f;
-        ctl.trace = nullptr;
-        if (sh.trace && b.rec_every > 0 && it % b.rec_every == 0) {
-            const int slot = it / b.rec_every - 1;
-            if (slot >= 0 && slot < b.rec_slots)
-                ctl.trace = sh.trace + (long long)slot * (sh.sx - 2 * b.bord) * (sh.sy - 2 * b.bord);
-        }
-        const int l = t - sh.item0;
-        const int tix = l % sh.ntx;
-        const int rest = l / sh.ntx;
-        step_tile<TY, CM, false>(a, ctl, tix, rest % sh.nty, rest / sh.nty, s);
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    if (t >= b.nitems) return;
+    int lo = 0, hi = b.nshots - 1;  // shot owning item t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.shots[mid].item0 <= t) lo = mid; else hi = mid - 1;
     }
+    const ShotDev &sh = b.shots[lo];
+    StepArgs a;
+    a.prev = sh.lv[b.parity];
+    a.cur = sh.lv[1 - b.parity];
+    a.coef = sh.coef;
+    a.dvx = a.dvy = nullptr;
+    a.dxxinv = b.dxxinv; a.dyyinv = b.dyyinv; a.dzzinv = b.dzzinv;
+    a.dxinv = b.dxinv; a.dyinv = b.dyinv; a.dzinv = b.dzinv; a.dt = b.dt;
+    a.sx = sh.sx; a.sy = sh.sy; a.sz = sh.sz; a.bord = b.bord; a.px = sh.px;
+    a.ntx = sh.ntx; a.nty = sh.nty; a.zt = b.zt; a.nblk = sh.nitems;
+    a.rec_iz = sh.rec_iz;
+    a.src_idx = sh.src_idx;
+    StepCtl ctl;
+    ctl.do_src = b.src_tab && (k + 1) < b.src_n;
+    ctl.src_add = ctl.do_src ? b.src_tab[k + 1] : 0.0f;
+    ctl.trace = nullptr;
+    if (sh.trace && b.rec_every > 0 && it % b.rec_every == 0) {
+        const int slot = it / b.rec_every - 1;
+        if (slot >= 0 && slot < b.rec_slots)
+            ctl.trace = sh.trace + (long long)slot * (sh.sx - 2 * b.bord) * (sh.sy - 2 * b.bord);
+    }
+    const int l = t - sh.item0;
+    const int tix = l % sh.ntx;
+    const int rest = l / sh.ntx;
+    step_tile<TY, CM, false>(a, ctl, tix, rest % sh.nty, rest / sh.nty, s);
 }
 
 }  // namespace fl

@@ -89,7 +89,7 @@ long long fl_device_bytes(void);   /* device memory held by the back-end */
 /* Algorithmic HBM bytes per updated cell per step of the current configuration. */
 int fl_bytes_per_cell(void);
 
-/* ---------------- batched shots: one persistent launch over (shot, tile) items -------- */
+/* ---------------- batched shots: one launch per step over (shot, tile) items ---------- */
 typedef struct {
     int sx, sy, sz;   /* array extents of the shot's sub-grid, halo and absorbing zone included */
     int isource;      /* source cell index within the sub-grid */

@@ -21,8 +21,11 @@ sources) timed on a few steps of the same grid on the host cores.
 
 extras (skipped with --quick), each timed over its whole run with inputs resident:
   twopass   the der1der1 two-pass variant on the same grid (104 algorithmic B/sample)
-  mixed     BF16 / FP16 coefficients and the FP16 in-kernel recompute on the same grid,
-            receivers on plane z=28 every 4 steps, 50-step graph chunks
+  fp32_graph / mixed   FP32, BF16 / FP16 coefficients and the FP16 in-kernel recompute on
+            the same grid, receivers on plane z=28 every 4 steps, 50-step graph chunks
+  rel_l2_vs_fp64   relative L2 (in FP64) of the receiver traces and the final pc against
+            the device FP64 reference (fl_ref64_*: the der1der1 loop in double, pinned
+            bitwise to the FP64 oracle by the tests) on the same grid and steps
   batched   32 shots on uneven (128..256)^2 x 320 apertures cut from one seeded
             512x512x320 model (absorb 16), 800 steps in 50-step graphs, autotuned
             tiles, FP32, BF16 and FP16-recompute coefficients; receivers every 4 steps
@@ -117,14 +120,38 @@ def dropin(F, np, model, shape, dt, isrc, args):
     avg_ms = kms / max(kcount, 1)
     r = {"value": 1e-6 * samples * args.steps / (region_ms * 1e-3), "region_ms": region_ms,
          "avg_kernel_ms": avg_ms, "launches": kcount, "bpc": bpc, "tiles": (ty, zt),
-         "achieved": bpc * samples / (avg_ms * 1e-3) / 1e9, "finite": finite}
+         "achieved": bpc * samples / (avg_ms * 1e-3) / 1e9, "finite": finite, "pc": pc}
     log("drop-in %s/%s: region %.1f ms (host wall %.1f ms), kernel avg %.4f ms over %d, %.0f GB/s, "
         "%.0f MS/s, finite=%s" % (args.variant, args.precision, region_ms, wall * 1e3, avg_ms, kcount,
                                   r["achieved"], r["value"], finite))
     return r
 
 
-def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, tiles):
+def rel_l2(np, a, b):
+    """Relative L2 of a against b, accumulated in FP64."""
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.sqrt(np.dot(a - b, a - b) / np.dot(b, b)))
+
+
+def fp64_reference(F, np, model, shape, dt, isrc, steps, extra, rec_iz):
+    """The device FP64 reference (der1der1 in double, exact weights): traces every 4 steps
+    and pc after `steps` steps, then pc after `steps + extra`."""
+    t0 = time.time()
+    r = F.Ref64(shape, D, dt, model)
+    try:
+        src = F.sources(dt, steps + extra)
+        r.run(steps, isrc, src=src[:steps], rec_iz=rec_iz, rec_every=4, max_traces=(steps + 3) // 4)
+        out = {"traces": r.traces(), "pc_steps": r.field("pc")}
+        r.run(extra, isrc, it0=steps, src=src[steps:])
+        out["pc_total"] = r.field("pc")
+    finally:
+        r.destroy()
+    log("FP64 reference: %d + %d steps in %.1f s" % (steps, extra, time.time() - t0))
+    return out
+
+
+def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, tiles, ref=None):
     """steps time steps in 50-step graph chunks, receivers every 4 steps; whole-run timing."""
     samples = (shape[0] - 8) * (shape[1] - 8) * (shape[2] - 8)
     d = F.Driver(variant=variant, precision=precision, sync=False)
@@ -146,6 +173,9 @@ def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, 
     r = {"value": round(value, 1), "ms_per_step": round(region_ms / (steps - 2), 4),
          "bytes_per_sample": bpc, "achieved_GBs": round(bpc * value * 1e-3, 1),
          "frac": round(bpc * value * 1e-3 / HBM_PEAK_GBS, 4), "tiles": [ty, zt], "finite": finite}
+    if ref is not None:
+        r["rel_l2_vs_fp64"] = {"traces": round(rel_l2(np, tr, ref["traces"]), 9),
+                               "pc": round(rel_l2(np, pc, ref["pc_steps"]), 9)}
     log("%s/%s graph run: %s" % (variant, precision, r))
     return r
 
@@ -224,15 +254,23 @@ def main():
         sys.exit(1)
     value = main_r["value"]
 
+    pc_main = main_r.pop("pc")
     extras = None
     if not args.quick:
-        extras = {"twopass": graph_run(F, np, model, shape, dt, isrc, "twopass", "fp32", args.steps,
-                                       rec_iz, "auto"),
+        ref = fp64_reference(F, np, model, shape, dt, isrc, args.steps, args.warmup, rec_iz)
+        main_err = rel_l2(np, pc_main, ref["pc_total"])
+        del pc_main
+        extras = {"fp32_dropin_rel_l2_vs_fp64": {"pc": round(main_err, 9)},
+                  "twopass": graph_run(F, np, model, shape, dt, isrc, "twopass", "fp32", args.steps,
+                                       rec_iz, "auto", ref),
+                  "fp32_graph": graph_run(F, np, model, shape, dt, isrc, "fused", "fp32", args.steps,
+                                          rec_iz, "auto", ref),
                   "mixed": {}}
         for prec in ("bf16", "fp16", "fp16r"):
-            r = graph_run(F, np, model, shape, dt, isrc, "fused", prec, args.steps, rec_iz, "auto")
+            r = graph_run(F, np, model, shape, dt, isrc, "fused", prec, args.steps, rec_iz, "auto", ref)
             r["vs_fp32_dropin"] = round(r["value"] / value, 3)
             extras["mixed"][prec] = r
+        del ref
         bt = batched(F, np, ("fp32", "bf16", "fp16r"), args.batch_steps, args.shots)
         for prec in ("fp32", "bf16", "fp16r"):
             bt[prec]["vs_fp32_dropin"] = round(bt[prec]["value"] / value, 3)

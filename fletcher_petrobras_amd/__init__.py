@@ -90,6 +90,11 @@ def lib():
         "fl_batch_autotune": (i, [i, C.c_char_p]),
         "fl_batch_kernel_ms": (i, [C.POINTER(d)]),
         "fl_batch_destroy": (None, []),
+        "fl_ref64_create": (i, [i, i, i, i, f, f, f, f] + [_f32p] * 6),
+        "fl_ref64_run": (i, [i, i, i, _f32p, i, i, i]),
+        "fl_ref64_get_field": (i, [i, _f64p]),
+        "fl_ref64_get_traces": (i, [_f64p, i]),
+        "fl_ref64_destroy": (None, []),
         # host model setup (fl_model.h)
         "fl_rand_seed": (None, [vp, C.c_uint32]),
         "fl_rand_next": (C.c_int32, [vp]),
@@ -456,3 +461,37 @@ class Batch:
 
     def destroy(self):
         self.L.fl_batch_destroy()
+
+
+class Ref64:
+    """FP64 accuracy reference on the device (fl_ref64_*): the der1der1 time loop in double
+    with exact weights and FP64 coefficients, reference layout.  One instance at a time."""
+
+    def __init__(self, shape, d, dt, model, bord=BORD):
+        self.L = lib()
+        self.shape, self.bord, self.dt = tuple(shape), bord, dt
+        m = [_c32(a) for a in model]
+        sz, sy, sx = self.shape
+        _check(self.L.fl_ref64_create(sx, sy, sz, bord, d[0], d[1], d[2], dt, *m))
+        self.slots = 0
+
+    def run(self, st, isource, it0=0, src=None, rec_iz=-1, rec_every=0, max_traces=0):
+        if src is None:
+            src = sources(self.dt, st, it0)
+        _check(self.L.fl_ref64_run(it0, st, isource, _c32(src), rec_iz, rec_every, max_traces))
+        if rec_every:
+            self.slots = max_traces
+
+    def field(self, which):
+        out = np.empty(self.shape, np.float64)
+        _check(self.L.fl_ref64_get_field(FIELD[which], out))
+        return out
+
+    def traces(self):
+        sz, sy, sx = self.shape
+        out = np.empty((self.slots, sy - 2 * self.bord, sx - 2 * self.bord), np.float64)
+        n = _check(self.L.fl_ref64_get_traces(out, self.slots))
+        return out[:n]
+
+    def destroy(self):
+        self.L.fl_ref64_destroy()

@@ -872,3 +872,4 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
 }
 
 #include "fl_batch.inc"
+#include "fl_ref64.inc"

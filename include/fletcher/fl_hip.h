@@ -110,6 +110,25 @@ int fl_batch_autotune(int steps, const char *cache_path);
 int fl_batch_kernel_ms(double *ms); /* device time of the last fl_batch_run */
 void fl_batch_destroy(void);
 
+/* ---------------- FP64 accuracy reference on the device ----------------
+ * No reference counterpart as a back-end: it is the reference's der1der1 time loop
+ * (der1der1/OpenMP/openmp_propagate.c:9-95, der1der1/sample.h:35-123, model.c:71-88)
+ * evaluated in FP64 with exact rational weights and FP64 coefficients
+ * (fl_precompute_f64), so that full-size runs of the FP32 and 16-bit forms have an
+ * FP64 result to be measured against.  Reference layout, one state per process,
+ * independent of the DRIVER_* and fl_batch_* state. */
+int fl_ref64_create(int sx, int sy, int sz, int bord, float dx, float dy, float dz, float dt,
+                    const float *vpz, const float *vsv, const float *epsilon, const float *delta,
+                    const float *phi, const float *theta);
+/* Steps it0+1..it0+nsteps: before step it, src[it-1-it0] (FP32 Ricker samples, promoted)
+ * is added to pc and qc at iSource; pc on plane rec_iz is kept after every step with
+ * it % rec_every == 0 (rec_every 0: none) in up to max_traces slots. */
+int fl_ref64_run(int it0, int nsteps, int iSource, const float *src, int rec_iz, int rec_every,
+                 int max_traces);
+int fl_ref64_get_field(int which, double *host);   /* 0 pp, 1 pc, 2 qp, 3 qc */
+int fl_ref64_get_traces(double *host, int max_traces);  /* returns the slot count filled */
+void fl_ref64_destroy(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -205,3 +205,13 @@ def test_pack_coefs16_matches_fp64_precompute(fl):
                 assert np.allclose(got, ref, rtol=2 ** -8, atol=1e-30)
             else:
                 assert np.array_equal(out[k], ref.astype(np.float16).view(np.uint16))
+
+
+def test_precompute_f64_equals_oracle_f64(fl, oracle):
+    """The device FP64 reference takes its coefficients from fl_precompute_f64; the FP64
+    oracle from oracle_precompute_f64: the same formulas, bitwise."""
+    m = fl.model_gradient((18, 16, 14), 4, seed=5)
+    a = fl.precompute(m, "f64")
+    b = oracle.precompute(m, "f64")
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert np.array_equal(x, y), fl.COEF_NAMES[k]

@@ -22,6 +22,8 @@
 
 using namespace fl;
 
+extern "C" int fl_autotune(int steps, const char *cache_path);
+
 // ------------------------------------------------------------------ errors
 static char g_err[512] = "";
 static int g_status = FL_OK;
@@ -430,6 +432,19 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
     return FL_OK;
 }
 
+// Selection through the environment for front-ends that only call driver.h (the
+// reference's main.c/model.c): FL_VARIANT = fused|direct|twopass, FL_PRECISION =
+// fp32|bf16|fp16|fp16r, applied at DRIVER_Initialize when set; FL_TUNE = cache file, the
+// autotuner runs (or reads the cache) right after it.
+static int env_pick(const char *name, const char *const *opts, int n)
+{
+    const char *v = getenv(name);
+    if (!v || !*v) return -1;
+    for (int i = 0; i < n; i++)
+        if (strcmp(v, opts[i]) == 0) return i;
+    return -2;
+}
+
 extern "C" void DRIVER_Initialize(const int sx, const int sy, const int sz, const int bord, float dx,
                                   float dy, float dz, float dt, float *__restrict__ vpz,
                                   float *__restrict__ vsv, float *__restrict__ epsilon,
@@ -438,8 +453,20 @@ extern "C" void DRIVER_Initialize(const int sx, const int sy, const int sz, cons
                                   float *__restrict__ pc, float *__restrict__ qp,
                                   float *__restrict__ qc)
 {
-    if (initialize(sx, sy, sz, bord, dx, dy, dz, dt, vpz, vsv, epsilon, delta, phi, theta, pp, pc, qp, qc) == FL_OK)
-        fl_clear_error();
+    static const char *const variants[] = {"fused", "direct", "twopass"};
+    static const char *const precs[] = {"fp32", "bf16", "fp16", "fp16r"};
+    const int v = env_pick("FL_VARIANT", variants, 3), p = env_pick("FL_PRECISION", precs, 4);
+    if (v == -2 || p == -2) {
+        fail(FL_ERR_ARG, "FL_VARIANT / FL_PRECISION hold an unknown value");
+        return;
+    }
+    if (v >= 0) g_variant = v;
+    if (p >= 0) g_precision = p;
+    if (initialize(sx, sy, sz, bord, dx, dy, dz, dt, vpz, vsv, epsilon, delta, phi, theta, pp, pc, qp, qc) != FL_OK)
+        return;
+    fl_clear_error();
+    const char *tune = getenv("FL_TUNE");
+    if (tune && *tune) fl_autotune(5, tune);
 }
 
 extern "C" void DRIVER_Finalize() { free_device(); }

@@ -8,7 +8,7 @@
  * Back-end options come from the environment (the reference has none):
  *   FL_VARIANT=fused|direct|twopass   FL_PRECISION=fp32|bf16|fp16|fp16r
  *   FL_RSF=<name>  dump pc every 0.01 s of simulated time as <name>.rsf (utils.c layout)
- *   FL_TUNE=<cache file>  autotune tiles for this grid before the run
+ *   FL_TUNE=<cache file>  autotune tiles for this grid at DRIVER_Initialize
  * Errors are reported on stderr with a non-zero exit status.
  */
 #include <math.h>
@@ -92,15 +92,9 @@ int main(int argc, char **argv)
         rsf = fl_rsf_open(rsfname, 0, sx - 1, 0, sy - 1, 0, sz - 1, dx, dy, dz, dt);
         if (!rsf) { fprintf(stderr, "cannot open %s.rsf\n", rsfname); return 3; }
     }
-    const char *tune = getenv("FL_TUNE");
+    /* FL_TUNE is applied by DRIVER_Initialize itself (the library reads the environment) */
     fl_report_t rep;
     float *pp = f[0], *pc = f[1], *qp = f[2], *qc = f[3];
-    if (tune && *tune) {
-        /* initialise once to tune, then let the run re-initialise on the tuned tiles */
-        DRIVER_Initialize(sx, sy, sz, bord, dx, dy, dz, dt, m[0], m[1], m[2], m[3], m[4], m[5], pp, pc, qp, qc);
-        if (fl_status() == FL_OK) fl_autotune(5, tune);
-        DRIVER_Finalize();
-    }
     fl_model_run(st, iSource, dtOutput, sx, sy, sz, bord, dx, dy, dz, dt, &pp, &pc, &qp, &qc, m[0], m[1],
                  m[2], m[3], m[4], m[5], rsf, 0, &rep);
     if (rsf) fl_rsf_close(rsf);

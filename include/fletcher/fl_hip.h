@@ -38,13 +38,16 @@ const char *fl_last_error(void);
 int fl_status(void);      /* code of the last failed call, FL_OK after fl_clear_error */
 void fl_clear_error(void);
 
+/* Front-ends that call only driver.h select with the environment instead, read by
+ * DRIVER_Initialize: FL_VARIANT=fused|direct|twopass, FL_PRECISION=fp32|bf16|fp16|fp16r,
+ * FL_TUNE=<cache file> (autotune right after initialisation). */
 int fl_set_variant(int variant);   /* takes effect at the next step */
 int fl_get_variant(void);
 int fl_set_precision(int precision); /* used by the next DRIVER_Initialize */
 int fl_get_precision(void);
 int fl_set_sync(int on);           /* 1 (default): DRIVER_Propagate waits for the step */
 
-/* Tile shape of the z-marching kernel: ty rows (4, 8 or 16) of 64 cells, z chunks of
+/* Tile shape of the z-marching kernel: ty rows (4, 8 or 16) of 32 cells, z chunks of
  * zt planes.  0 selects the built-in default for the grid. */
 int fl_set_tiles(int ty, int zt);
 int fl_get_tiles(int *ty, int *zt);

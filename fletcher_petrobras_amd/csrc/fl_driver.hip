@@ -821,8 +821,9 @@ static std::string tune_key(const char *kind, int sx, int sy, int sz, int cm)
 
 static std::vector<int> zt_candidates(int nzu)
 {
-    // fixed depths plus even splits of the column into 2..16 chunks
-    std::vector<int> c = {16, 24, 32, 48, 64, 96, 128, 192};
+    // fixed depths (4 + a multiple of 10 runs every output plane of a chunk on the
+    // statically addressed steps) plus even splits of the column into 2..16 chunks
+    std::vector<int> c = {14, 24, 34, 44, 54, 64, 74, 84, 94, 124, 154, 194};
     for (int k : {2, 4, 6, 8, 12, 16}) c.push_back((nzu + k - 1) / k);
     std::vector<int> r;
     for (int z : c)

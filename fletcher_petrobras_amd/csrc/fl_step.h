@@ -44,10 +44,23 @@ __device__ __forceinline__ float ld_f32(const void *base, uint32_t off)
     return *(const float *)((const char *)base + off);
 }
 
-// 1: the steady steps load the next output plane's coefficients and pp/qp one step ahead.
-#ifndef FL_PREFETCH
-#define FL_PREFETCH 1
+// Prefetch distances in steps: the output plane's coefficient record and previous-level
+// value (prefetch_steps, 0 = loaded in their own step) and the staged input plane
+// (FL_STAGE_AHEAD, 1 or 2).  Both use two register sets chosen by step parity.
+// Measured at 344^3: two steps ahead gain 3% (FP32) and 7% (FP16 recompute); the stored
+// 16-bit forms, at 80 VGPRs, lose 6% and keep one; staging two planes ahead gains nothing.
+#ifndef FL_STAGE_AHEAD
+#define FL_STAGE_AHEAD 1
 #endif
+template <int CM>
+constexpr int prefetch_steps()
+{
+#ifdef FL_PREFETCH
+    return FL_PREFETCH;
+#else
+    return (CM == CM_BF16 || CM == CM_F16) ? 1 : 2;
+#endif
+}
 
 // Lane byte offset beyond every buffer range used here (spans are < 2^31 bytes).
 constexpr uint32_t FL_OOB = 0x80000000u;
@@ -67,7 +80,7 @@ __device__ __forceinline__ void bstore32(__amdgpu_buffer_rsrc_t r, uint32_t voff
 }
 
 // Coefficient record of one cell as loaded: FP32 10 values (40 B); BF16 / FP16 10 values
-// (20 B); FP16 recompute inputs 6 values (12 B).  Kept raw until used, so the loads stay
+// (20 B); FP16 recompute inputs 6 values (12 B: the 4 velocity factors and 2 angles).  Kept raw until used, so the loads stay
 // in flight across barriers.
 template <int CM>
 struct CoefRec {
@@ -113,17 +126,17 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, Coef &k)
             k.c[2 * j] = f16_to_f32((uint16_t)r.w[j]);
             k.c[2 * j + 1] = f16_to_f32((uint16_t)(r.w[j] >> 16));
         }
-    } else {  // CM_F16R: rebuild from vpz*dt, vsv*dt, eps, delta, theta, phi-pi
+    } else {  // CM_F16R: v2px, v2pz, v2sz, v2pn (times dt^2), theta/pi, (phi-pi)/pi
         float v[6];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             v[2 * j] = f16_to_f32((uint16_t)r.w[j]);
             v[2 * j + 1] = f16_to_f32((uint16_t)(r.w[j] >> 16));
         }
-        const float w2 = v[0] * v[0];
+        // the angular coefficients of precomp.h:25-38 from the angles
         float st, ct, spc, cpc;
-        __sincosf(v[4], &st, &ct);
-        __sincosf(v[5], &spc, &cpc);
+        __sincosf(v[4] * 3.14159265358979f, &st, &ct);
+        __sincosf(v[5] * 3.14159265358979f, &spc, &cpc);
         const float sp = -spc, cp = -cpc;  // phi = phic + pi
         const float st2 = st * st;
         const float s2t = 2.0f * st * ct;
@@ -134,10 +147,10 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, Coef &k)
         k.c[3] = st2 * s2p;
         k.c[4] = s2t * sp;
         k.c[5] = s2t * cp;
-        k.c[6] = w2 * (1.0f + 2.0f * v[2]);
-        k.c[7] = w2;
-        k.c[8] = v[1] * v[1];
-        k.c[9] = w2 * (1.0f + 2.0f * v[3]);
+        k.c[6] = v[0];
+        k.c[7] = v[1];
+        k.c[8] = v[2];
+        k.c[9] = v[3];
     }
 }
 
@@ -219,17 +232,20 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         const bool ok = hk[k] && gx >= 0 && gx < a.sx && gy < a.sy;
         hoff[k] = ok ? ((uint32_t)(gy * a.px + gx) * 2u + (uint32_t)(w & 1)) * 4u : FL_OOB;
     }
-    float pre[NLD];
-    auto fetch = [&](int i) {  // raw loads of input plane f0 + i
+    constexpr int SA = FL_STAGE_AHEAD;
+    float pre[SA][NLD];  // raw loads of plane j in set j % SA
+    auto fetch = [&](auto Sc, int i) {  // loads of input plane f0 + i into set Sc
+        constexpr int S = decltype(Sc)::value;
         const uint32_t so = (uint32_t)i * pb8;
 #pragma unroll
         for (int k = 0; k < NLD; k++)
-            if (hk[k]) pre[k] = __builtin_bit_cast(float, bload32(r_cur, hoff[k], so));
+            if (hk[k]) pre[S][k] = __builtin_bit_cast(float, bload32(r_cur, hoff[k], so));
     };
-    auto stage = [&](float *T) {
+    auto stage = [&](auto Sc, float *T) {
+        constexpr int S = decltype(Sc)::value;
 #pragma unroll
         for (int k = 0; k < NLD; k++)
-            if (hk[k]) T[wv * 64 + k * NT + lane] = pre[k];
+            if (hk[k]) T[wv * 64 + k * NT + lane] = pre[S][k];
     };
 
     // lane offsets in the LDS planes (window corners: every stencil read is a
@@ -261,10 +277,11 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     CoefRec<CM> kp[2];
     float ppv[2] = {0.0f, 0.0f};
 
-    // prologue: plane f0 into ring slot 0, loads of plane f0+1 in flight
-    fetch(0);
-    stage(s.t[0]);
-    fetch(1);
+    // prologue: plane f0 into ring slot 0, the next SA planes in flight
+    fetch(ic<0>{}, 0);
+    stage(ic<0>{}, s.t[0]);
+    fetch(ic<1 % SA>{}, 1);
+    if constexpr (SA == 2) fetch(ic<0>{}, 2);
     __syncthreads();
 
     int r6 = 0, r5 = 0;  // LDS ring slots of plane f0 + i: i % 6, i % 5
@@ -273,15 +290,18 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     //   XW   Dx halo planes 0 never / 1 always / 2 if i < nz+4 (f is an output plane)
     //   OUT  output plane z = f-4
     //   GEN  generic copy: plane i-8+k in slot k, queues shifted afterwards
-    //   PF   bit 0: this step's output streams were loaded by the previous step into set
-    //        i % 2; bits 1-2: load those of the next step into set (i+1) % 2 (2 always,
+    //   PF   bit 0: this step's output streams were loaded PD steps earlier into set
+    //        i % 2; bits 1-2: load those of step i + PD into set (i + PD) % 2 (2 always,
     //        4 if `pf_next`)
-    auto step = [&](auto PHc, auto XWc, auto OUTc, auto GENc, auto PFc, const int i, const bool pf_next) {
+    //   PAR  i % 2 (the register sets of the step)
+    auto step = [&](auto PHc, auto XWc, auto OUTc, auto GENc, auto PFc, auto PARc, const int i,
+                    const bool pf_next) {
         constexpr int PH = decltype(PHc)::value;
         constexpr int XW = decltype(XWc)::value;
         constexpr bool OUT = decltype(OUTc)::value != 0, GEN = decltype(GENc)::value != 0;
         constexpr int PF = decltype(PFc)::value;
-        constexpr int SET = PH & 1;  // = i % 2 in the static steps
+        constexpr int SET = decltype(PARc)::value;
+        constexpr int PD = prefetch_steps<CM>() > 0 ? prefetch_steps<CM>() : 1;
         const int f = f0 + i;
         const int z = f - 4;
         const int r6n = r6 == 5 ? 0 : r6 + 1, r5n = r5 == 4 ? 0 : r5 + 1;
@@ -352,9 +372,9 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             dyy = der2_lds(zp, RW, a.dyyinv);
             dxy = der1_lds(Xz + oxw + CX, XW2, a.dyinv);
         }
-        // next plane into the ring, the one after it in flight
-        if (i + 1 <= nz + 7) stage(s.t[r6n]);
-        if (i + 2 <= nz + 7) fetch(i + 2);
+        // next plane into the ring from its set, then the set refilled SA planes ahead
+        if (i + 1 <= nz + 7) stage(ic<(SET + 1) % SA>{}, s.t[r6n]);
+        if (i + 1 + SA <= nz + 7) fetch(ic<(SET + 1) % SA>{}, i + 1 + SA);
 
         if constexpr (OUT) {
             constexpr int b = GEN ? 0 : (PH + 2) % 10;  // slot of plane i-8
@@ -405,9 +425,10 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         }
         if constexpr ((PF & 2) != 0 || (PF & 4) != 0) {
             if ((PF & 2) || pf_next) {
-                load_rec<CM>(span_rsrc(cpl, pbr), vor, kp[SET ^ 1]);
+                constexpr int NS2 = (SET + PD) & 1;
+                load_rec<CM>(span_rsrc(cpl, pbr), vor, kp[NS2]);
                 cpl += pbr;
-                ppv[SET ^ 1] = __builtin_bit_cast(float, bload32(r_prev, vo8, (uint32_t)(i - 7) * pb8));
+                ppv[NS2] = __builtin_bit_cast(float, bload32(r_prev, vo8, (uint32_t)(i + PD - 8) * pb8));
             }
         }
         if constexpr (GEN) {
@@ -423,32 +444,39 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         __syncthreads();
     };
 
-    // planes 0-3: queues only; 4-7: queues + Dx halo planes (when an output plane); plane 7
-    // loads the streams of the first output plane when a static group follows
+    // planes 0-3: queues only; 4-7: queues + Dx halo planes (when an output plane); planes
+    // 8-PD .. 7 load the streams of the first output planes when a static group follows
+    constexpr int PD = prefetch_steps<CM>() > 0 ? prefetch_steps<CM>() : 1;
+    constexpr int PFB = prefetch_steps<CM>() ? 4 : 0;
     const bool grp0 = 8 + 9 < nz + 4;
-    step(ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 0, false);
-    step(ic<1>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 1, false);
-    step(ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 2, false);
-    step(ic<3>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 3, false);
-    step(ic<4>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, 4, false);
-    step(ic<5>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, 5, false);
-    step(ic<6>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, 6, false);
-    step(ic<7>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<FL_PREFETCH ? 4 : 0>{}, 7, grp0);
+    step(ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 0, false);
+    step(ic<1>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<1>{}, 1, false);
+    step(ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 2, false);
+    step(ic<3>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<1>{}, 3, false);
+    step(ic<4>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 4, false);
+    step(ic<5>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<1>{}, 5, false);
+    step(ic<6>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<(PD == 2 ? PFB : 0)>{}, ic<0>{}, 6, grp0);
+    step(ic<7>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<PFB>{}, ic<1>{}, 7, grp0);
     // steady state: groups of 10 planes (i = 8 mod 10 at their start), each writing Dx
-    // halos and an output plane, with the output streams one step ahead
+    // halos and an output plane, with the output streams PD steps ahead
     int i = 8;
     for (; i + 9 < nz + 4; i += 10) {
         const bool more = i + 10 + 9 < nz + 4;
         unroll_phases(
             [&](auto P) {
                 constexpr int p = decltype(P)::value;
-                constexpr int pf = FL_PREFETCH ? (1 | (p < 9 ? 2 : 4)) : 0;
-                step(ic<(8 + p) % 10>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<pf>{}, i + p, more);
+                constexpr int pf = prefetch_steps<CM>() ? (1 | (p + PD <= 9 ? 2 : 4)) : 0;
+                step(ic<(8 + p) % 10>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<pf>{}, ic<p & 1>{}, i + p, more);
             },
             std::make_integer_sequence<int, 10>{});
     }
-    // the rest, queues shifted after each plane
-    for (; i <= nz + 7; i++) step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, i, false);
+    // the rest (i even here), queues shifted after each plane, two steps per round so that
+    // the register sets keep their parity
+    for (; i + 1 <= nz + 7; i += 2) {
+        step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<0>{}, i, false);
+        step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<1>{}, i + 1, false);
+    }
+    if (i <= nz + 7) step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<0>{}, i, false);
 }
 
 // Waves per SIMD the register allocator is asked for.  With 8-row tiles the LDS rings

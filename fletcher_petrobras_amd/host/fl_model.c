@@ -293,15 +293,18 @@ void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const fl
                          const float *delta, const float *phi, const float *theta, double dt,
                          uint16_t *const *out)
 {
-    const double pi = 3.14159265358979323846;
+    const double pi = 3.14159265358979323846, dt2 = dt * dt;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
-        out[0][i] = fl_f64_to_f16((double)vpz[i] * dt);
-        out[1][i] = fl_f64_to_f16((double)vsv[i] * dt);
-        out[2][i] = fl_f64_to_f16((double)epsilon[i]);
-        out[3][i] = fl_f64_to_f16((double)delta[i]);
-        out[4][i] = fl_f64_to_f16((double)theta[i]);
-        out[5][i] = fl_f64_to_f16((double)phi[i] - pi);
+        /* the four velocity factors of fl_precompute_f64, times dt^2 */
+        const double v2pz = (double)vpz[i] * vpz[i];
+        out[0][i] = fl_f64_to_f16(v2pz * (1.0 + 2.0 * epsilon[i]) * dt2);
+        out[1][i] = fl_f64_to_f16(v2pz * dt2);
+        out[2][i] = fl_f64_to_f16((double)vsv[i] * vsv[i] * dt2);
+        out[3][i] = fl_f64_to_f16(v2pz * (1.0 + 2.0 * delta[i]) * dt2);
+        /* angles in units of pi: theta/pi in [0, 1/3], (phi - pi)/pi in [-1, 1) */
+        out[4][i] = fl_f64_to_f16((double)theta[i] / pi);
+        out[5][i] = fl_f64_to_f16(((double)phi[i] - pi) / pi);
     }
 }
 

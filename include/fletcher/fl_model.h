@@ -62,7 +62,9 @@ void fl_precompute_f64(int64_t n, const float *vpz, const float *vsv, const floa
 void fl_pack_coefs16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                      const float *delta, const float *phi, const float *theta, double dt, int bf16,
                      uint16_t *const *out);
-/* Inputs of the in-kernel recompute, FP16: vpz*dt, vsv*dt, eps, delta, theta, phi-pi. */
+/* Inputs of the in-kernel recompute, FP16, each rounded once from FP64: the velocity
+ * factors v2px, v2pz, v2sz, v2pn of fl_precompute_f64 times dt^2, theta/pi and
+ * (phi-pi)/pi (the six angular coefficients are rebuilt from the angles per cell). */
 void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, double dt,
                          uint16_t *const *out);

@@ -215,3 +215,18 @@ def test_precompute_f64_equals_oracle_f64(fl, oracle):
     b = oracle.precompute(m, "f64")
     for k, (x, y) in enumerate(zip(a, b)):
         assert np.array_equal(x, y), fl.COEF_NAMES[k]
+
+
+def test_pack_recompute16_inputs(fl):
+    """FP16 recompute inputs: the FP64 velocity factors times dt^2 and the angles in units of
+    pi, each rounded once (numpy's correctly rounded float64 -> float16)."""
+    m = fl.model_gradient((16, 14, 12), 4, seed=11)
+    n = m[0].size
+    dt = 8.0e-4
+    c64 = fl.precompute(m, "f64")
+    out = [np.empty(m[0].shape, np.uint16) for _ in range(6)]
+    fl.lib().fl_pack_recompute16(n, *m, dt, (C.c_void_p * 6)(*[o.ctypes.data for o in out]))
+    want = [c64[6] * dt * dt, c64[7] * dt * dt, c64[8] * dt * dt, c64[9] * dt * dt,
+            m[5].astype(np.float64) / np.pi, (m[4].astype(np.float64) - np.pi) / np.pi]
+    for k in range(6):
+        assert np.array_equal(out[k], want[k].astype(np.float16).view(np.uint16)), k

@@ -86,12 +86,12 @@ def tiles_for(d, spec):
     return ty, zt
 
 
-def dropin(F, np, model, shape, dt, isrc, args):
+def dropin(F, np, model, shape, dt, isrc, args, d3=D):
     """The reference loop through the drop-in entry points, per-launch HIP events."""
     samples = (shape[0] - 8) * (shape[1] - 8) * (shape[2] - 8)
     d = F.Driver(variant=args.variant, precision=args.precision, sync=False)
     t0 = time.time()
-    d.initialize(shape, D, dt, model)
+    d.initialize(shape, d3, dt, model)
     log("initialize (precompute + upload) %.1f s, device %s" % (time.time() - t0, F.device_name()))
     ty, zt = tiles_for(d, args.tiles)
     log("tiles ty=%d zt=%d" % (ty, zt))
@@ -134,16 +134,17 @@ def rel_l2(np, a, b):
     return float(np.sqrt(np.dot(a - b, a - b) / np.dot(b, b)))
 
 
-def fp64_reference(F, np, model, shape, dt, isrc, steps, extra, rec_iz):
+def fp64_reference(F, np, model, shape, dt, isrc, steps, extra, rec_iz, d3=D):
     """The device FP64 reference (der1der1 in double, exact weights): traces every 4 steps
     and pc after `steps` steps, then pc after `steps + extra`."""
     t0 = time.time()
-    r = F.Ref64(shape, D, dt, model)
+    r = F.Ref64(shape, d3, dt, model)
     try:
         src = F.sources(dt, steps + extra)
         r.run(steps, isrc, src=src[:steps], rec_iz=rec_iz, rec_every=4, max_traces=(steps + 3) // 4)
         out = {"traces": r.traces(), "pc_steps": r.field("pc")}
-        r.run(extra, isrc, it0=steps, src=src[steps:])
+        if extra:
+            r.run(extra, isrc, it0=steps, src=src[steps:])
         out["pc_total"] = r.field("pc")
     finally:
         r.destroy()
@@ -151,11 +152,11 @@ def fp64_reference(F, np, model, shape, dt, isrc, steps, extra, rec_iz):
     return out
 
 
-def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, tiles, ref=None):
+def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, tiles, ref=None, d3=D):
     """steps time steps in 50-step graph chunks, receivers every 4 steps; whole-run timing."""
     samples = (shape[0] - 8) * (shape[1] - 8) * (shape[2] - 8)
     d = F.Driver(variant=variant, precision=precision, sync=False)
-    d.initialize(shape, D, dt, model)
+    d.initialize(shape, d3, dt, model)
     ty, zt = tiles_for(d, tiles)
     bpc = d.bytes_per_cell()
     d.record_traces(rec_iz, 4, (steps + 3) // 4)
@@ -215,6 +216,49 @@ def batched(F, np, precisions, steps, nshots):
     return out
 
 
+def large(F, np, args):
+    """The 1024x1024x768 model near HBM capacity: seeded layered model with 12 dipping layers
+    (vpz 1500-5000 m/s, eps in [0,0.3], delta in [0,0.2] <= eps, smooth theta and phi), absorb
+    32, dx=dy=dz=10 m (array 1096x1096x840), Courant 0.4.  FP32 drop-in and FP16 recompute,
+    each against the device FP64 reference over the whole grid."""
+    d3 = (10.0, 10.0, 10.0)
+    t0 = time.time()
+    model = F.model_layered((1024, 1024, 768), 32, seed=1024)
+    shape = model[0].shape
+    dt = F.cfl_dt(model, 10.0, 0.4)
+    rec_iz = 4 + 32 + 8
+    isrc = F.index(shape, shape[2] // 2, shape[1] // 2, rec_iz)
+    samples = (shape[0] - 8) * (shape[1] - 8) * (shape[2] - 8)
+    steps = args.large_steps
+    log("large model %s built in %.1f s, dt=%.6g" % (shape, time.time() - t0, dt))
+    ref = fp64_reference(F, np, model, shape, dt, isrc, steps, 0, rec_iz, d3)
+    main_r = dropin(F, np, model, shape, dt, isrc, argparse.Namespace(
+        variant="fused", precision="fp32", tiles=args.tiles, warmup=0, steps=steps), d3)
+    err = rel_l2(np, main_r.pop("pc"), ref["pc_steps"])
+    graphs = {}
+    for prec in ("fp32", "fp16r"):
+        graphs[prec] = graph_run(F, np, model, shape, dt, isrc, "fused", prec, steps, rec_iz, args.tiles, ref, d3)
+        graphs[prec]["vs_fp32_dropin"] = round(graphs[prec]["value"] / main_r["value"], 3)
+    del ref
+    value = main_r["value"]
+    line = {
+        "metric": "MSamples/s and % HBM peak, TTI 1024x1024x768: FP32 drop-in vs FP16 recompute",
+        "value": round(value, 1), "unit": "MSamples/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(main_r["region_ms"] / steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic seeded layered TTI model (12 dipping layers, smooth theta/phi, reference random boundary)",
+        "config": {"workload": "TTI 1024x1024x768 interior, absorb 32, array %dx%dx%d, dx=10 m, dt=%.6g s, "
+                               "fused variant, fp32 coefficients, tiles 32x%dx%d"
+                               % (shape[2], shape[1], shape[0], dt, main_r["tiles"][0], main_r["tiles"][1]),
+                   "samples_per_step": samples, "bytes_per_sample": main_r["bpc"]},
+        "roofline": {"bound": "hbm", "achieved": round(main_r["achieved"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(main_r["achieved"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "fp32_dropin_rel_l2_vs_fp64": {"pc": round(err, 9)},
+        "graph_runs": graphs,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +274,9 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline line only, no extras")
     ap.add_argument("--shots", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=800)
+    ap.add_argument("--large", action="store_true",
+                    help="only the 1024x1024x768 layered model (FP32 + FP16 recompute vs FP64)")
+    ap.add_argument("--large-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the Fletcher hot path here is single-GPU "
@@ -238,6 +285,9 @@ def main():
     import numpy as np
     import fletcher_petrobras_amd as F
 
+    if args.large:
+        large(F, np, args)
+        return
     n = args.nx
     t0 = time.time()
     model = F.model_gradient((n, n, n), args.absorb, seed=344)

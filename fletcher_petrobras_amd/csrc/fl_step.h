@@ -542,7 +542,11 @@ __global__ void batch_insert(const ShotDev *shots, int nshots, int parity, float
 template <int TY, int CM>
 constexpr int batch_waves()
 {
+#ifdef FL_BATCH_W4
+    return 4;
+#else
     return (TY == 8 && CM == CM_F16R) ? 6 : 4;
+#endif
 }
 template <int TY, int CM>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(batch_waves<TY, CM>())))

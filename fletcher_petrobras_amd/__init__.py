@@ -112,6 +112,7 @@ def lib():
         "fl_cfl_dt": (f, [C.c_int64, _f32p, _f32p, f, f]),
         "fl_f64_to_f16": (C.c_uint16, [d]),
         "fl_f64_to_bf16": (C.c_uint16, [d]),
+        "fl_angle_u16": (C.c_uint16, [d]),
         "fl_pack_coefs16": (None, [C.c_int64] + [_f32p] * 6 + [d, i, C.POINTER(vp)]),
         "fl_pack_recompute16": (None, [C.c_int64] + [_f32p] * 6 + [d, C.POINTER(vp)]),
         "fl_extract_window": (None, [_f32p] + [i] * 9 + [_f32p]),

@@ -80,7 +80,7 @@ __device__ __forceinline__ void bstore32(__amdgpu_buffer_rsrc_t r, uint32_t voff
 }
 
 // Coefficient record of one cell as loaded: FP32 10 values (40 B); BF16 / FP16 10 values
-// (20 B); FP16 recompute inputs 6 values (12 B: the 4 velocity factors and 2 angles).  Kept raw until used, so the loads stay
+// (20 B); FP16 recompute inputs (12 B: 4 FP16 velocity factors, 2 u16 angles).  Kept raw until used, so the loads stay
 // in flight across barriers.
 template <int CM>
 struct CoefRec {
@@ -126,18 +126,18 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, Coef &k)
             k.c[2 * j] = f16_to_f32((uint16_t)r.w[j]);
             k.c[2 * j + 1] = f16_to_f32((uint16_t)(r.w[j] >> 16));
         }
-    } else {  // CM_F16R: v2px, v2pz, v2sz, v2pn (times dt^2), theta/pi, (phi-pi)/pi
-        float v[6];
+    } else {  // CM_F16R: v2px, v2pz, v2sz, v2pn (times dt^2, FP16), theta, phi (u16 turns)
+        float v[4];
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
+        for (int j = 0; j < 2; j++) {
             v[2 * j] = f16_to_f32((uint16_t)r.w[j]);
             v[2 * j + 1] = f16_to_f32((uint16_t)(r.w[j] >> 16));
         }
-        // the angular coefficients of precomp.h:25-38 from the angles
-        float st, ct, spc, cpc;
-        __sincosf(v[4] * 3.14159265358979f, &st, &ct);
-        __sincosf(v[5] * 3.14159265358979f, &spc, &cpc);
-        const float sp = -spc, cp = -cpc;  // phi = phic + pi
+        // the angular coefficients of precomp.h:25-38; v_sin/v_cos take turns
+        const float tt = (float)(r.w[2] & 0xffffu) * (1.0f / 65536.0f);
+        const float tp = (float)(r.w[2] >> 16) * (1.0f / 65536.0f);
+        const float st = __builtin_amdgcn_sinf(tt), ct = __builtin_amdgcn_cosf(tt);
+        const float sp = __builtin_amdgcn_sinf(tp), cp = __builtin_amdgcn_cosf(tp);
         const float st2 = st * st;
         const float s2t = 2.0f * st * ct;
         const float s2p = 2.0f * sp * cp;

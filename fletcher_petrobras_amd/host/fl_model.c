@@ -289,6 +289,12 @@ void fl_pack_coefs16(int64_t n, const float *vpz, const float *vsv, const float 
     }
 }
 
+uint16_t fl_angle_u16(double turns)
+{
+    const double t = turns - floor(turns);           /* [0, 1) */
+    return (uint16_t)((uint32_t)nearbyint(t * 65536.0) & 0xffffu);
+}
+
 void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, double dt,
                          uint16_t *const *out)
@@ -302,9 +308,9 @@ void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const fl
         out[1][i] = fl_f64_to_f16(v2pz * dt2);
         out[2][i] = fl_f64_to_f16((double)vsv[i] * vsv[i] * dt2);
         out[3][i] = fl_f64_to_f16(v2pz * (1.0 + 2.0 * delta[i]) * dt2);
-        /* angles in units of pi: theta/pi in [0, 1/3], (phi - pi)/pi in [-1, 1) */
-        out[4][i] = fl_f64_to_f16((double)theta[i] / pi);
-        out[5][i] = fl_f64_to_f16(((double)phi[i] - pi) / pi);
+        /* angles as 16-bit fixed point in revolutions, round to nearest, modulo one turn */
+        out[4][i] = fl_angle_u16((double)theta[i] / (2.0 * pi));
+        out[5][i] = fl_angle_u16((double)phi[i] / (2.0 * pi));
     }
 }
 

@@ -30,9 +30,9 @@ enum { FL_VARIANT_FUSED = 0, FL_VARIANT_DIRECT = 1, FL_VARIANT_TWOPASS = 2 };
 
 /* Coefficient storage.  FP32 keeps the precomp.h volumes bit for bit; BF16/FP16
  * store the same 10 volumes (v2* pre-multiplied by dt^2, evaluated in FP64 and
- * rounded once); FP16_RECOMPUTE stores the 4 velocity factors (times dt^2) and
- * theta, phi in FP16 and rebuilds the 6 angular coefficients per cell.  Wavefields
- * stay FP32. */
+ * rounded once); FP16_RECOMPUTE stores the 4 velocity factors (times dt^2) in FP16
+ * and theta, phi as 16-bit fixed point turns, and rebuilds the 6 angular coefficients
+ * per cell.  Wavefields stay FP32. */
 enum { FL_PREC_FP32 = 0, FL_PREC_BF16 = 1, FL_PREC_FP16 = 2, FL_PREC_FP16_RECOMPUTE = 3 };
 
 const char *fl_last_error(void);

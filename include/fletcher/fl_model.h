@@ -62,9 +62,10 @@ void fl_precompute_f64(int64_t n, const float *vpz, const float *vsv, const floa
 void fl_pack_coefs16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                      const float *delta, const float *phi, const float *theta, double dt, int bf16,
                      uint16_t *const *out);
-/* Inputs of the in-kernel recompute, FP16, each rounded once from FP64: the velocity
- * factors v2px, v2pz, v2sz, v2pn of fl_precompute_f64 times dt^2, theta/pi and
- * (phi-pi)/pi (the six angular coefficients are rebuilt from the angles per cell). */
+/* Inputs of the in-kernel recompute, each rounded once from FP64: the velocity factors
+ * v2px, v2pz, v2sz, v2pn of fl_precompute_f64 times dt^2 in FP16, and theta, phi as
+ * 16-bit fixed point in revolutions (fl_angle_u16); the six angular coefficients are
+ * rebuilt from the angles per cell. */
 void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, double dt,
                          uint16_t *const *out);
@@ -75,6 +76,8 @@ void fl_interleave16(int64_t n, int nk, const uint16_t *const *src, uint16_t *ds
 /* Single conversions (round to nearest even), exported for tests. */
 uint16_t fl_f64_to_f16(double v);
 uint16_t fl_f64_to_bf16(double v);
+/* An angle given in turns as round(frac(turns) * 65536) mod 65536. */
+uint16_t fl_angle_u16(double turns);
 
 /* Number of time steps ceil(tmax/dt) in float, main.c:101. */
 int fl_num_steps(float tmax, float dt);

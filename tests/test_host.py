@@ -218,15 +218,22 @@ def test_precompute_f64_equals_oracle_f64(fl, oracle):
 
 
 def test_pack_recompute16_inputs(fl):
-    """FP16 recompute inputs: the FP64 velocity factors times dt^2 and the angles in units of
-    pi, each rounded once (numpy's correctly rounded float64 -> float16)."""
+    """FP16 recompute inputs: the FP64 velocity factors times dt^2, rounded once (numpy's
+    correctly rounded float64 -> float16), and the angles as u16 turns."""
     m = fl.model_gradient((16, 14, 12), 4, seed=11)
     n = m[0].size
     dt = 8.0e-4
     c64 = fl.precompute(m, "f64")
     out = [np.empty(m[0].shape, np.uint16) for _ in range(6)]
     fl.lib().fl_pack_recompute16(n, *m, dt, (C.c_void_p * 6)(*[o.ctypes.data for o in out]))
-    want = [c64[6] * dt * dt, c64[7] * dt * dt, c64[8] * dt * dt, c64[9] * dt * dt,
-            m[5].astype(np.float64) / np.pi, (m[4].astype(np.float64) - np.pi) / np.pi]
-    for k in range(6):
+    want = [c64[6] * dt * dt, c64[7] * dt * dt, c64[8] * dt * dt, c64[9] * dt * dt]
+    for k in range(4):
         assert np.array_equal(out[k], want[k].astype(np.float16).view(np.uint16)), k
+    for k, ang in ((4, m[5]), (5, m[4])):  # theta, phi as u16 turns
+        t = ang.astype(np.float64) / (2 * np.pi)
+        t = t - np.floor(t)
+        assert np.array_equal(out[k], (np.rint(t * 65536.0).astype(np.int64) & 0xFFFF).astype(np.uint16))
+        # round trip within half a step of the quantisation
+        back = out[k].astype(np.float64) / 65536.0 * 2 * np.pi
+        d = np.abs(np.angle(np.exp(1j * (back - ang.astype(np.float64)))))
+        assert d.max() <= np.pi / 65536 * (1 + 1e-9)

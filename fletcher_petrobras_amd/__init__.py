@@ -7,7 +7,8 @@ setup, which is the C library's too).
     Driver      the reference plug-in API (original/driver.h:8-22): initialize,
                 insert_source, propagate, update_pointers, finalize, plus the
                 fl_hip.h extensions (variants, precision, tiles, traces, graph runs)
-    Batch       many shots advanced by one persistent launch per step
+    Batch       many shots advanced by one launch per step over (shot, tile) items
+    Ref64       the FP64 device reference (der1der1 in double, exact weights)
     model_*     host model setup (main.c, boundary.c, precomp.h restated)
 
 Arrays are numpy float32 volumes shaped (sz, sy, sx), x fastest (map.h:14).

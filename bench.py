@@ -21,8 +21,9 @@ sources) timed on a few steps of the same grid on the host cores.
 
 extras (skipped with --quick), each timed over its whole run with inputs resident:
   twopass   the der1der1 two-pass variant on the same grid (104 algorithmic B/sample)
-  fp32_graph / mixed   FP32, BF16 / FP16 coefficients and the FP16 in-kernel recompute on
-            the same grid, receivers on plane z=28 every 4 steps, 50-step graph chunks
+  fp32_graph / mixed   FP32, the FP32 in-kernel recompute (fp32r: FP32 arithmetic, 24-byte
+            records), BF16 / FP16 coefficients and the FP16 in-kernel recompute on the
+            same grid, receivers on plane z=28 every 4 steps, 50-step graph chunks
   rel_l2_vs_fp64   relative L2 (in FP64) of the receiver traces and the final pc against
             the device FP64 reference (fl_ref64_*: the der1der1 loop in double, pinned
             bitwise to the FP64 oracle by the tests) on the same grid and steps
@@ -267,7 +268,7 @@ def main():
     ap.add_argument("--nx", type=int, default=344)
     ap.add_argument("--absorb", type=int, default=16)
     ap.add_argument("--variant", default="fused", choices=["fused", "direct", "twopass"])
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp16r"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp16r", "fp32r"])
     ap.add_argument("--tiles", default="auto", help="ty,zt or 'auto' (autotuner)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -316,7 +317,7 @@ def main():
                   "fp32_graph": graph_run(F, np, model, shape, dt, isrc, "fused", "fp32", args.steps,
                                           rec_iz, "auto", ref),
                   "mixed": {}}
-        for prec in ("bf16", "fp16", "fp16r"):
+        for prec in ("fp32r", "bf16", "fp16", "fp16r"):
             r = graph_run(F, np, model, shape, dt, isrc, "fused", prec, args.steps, rec_iz, "auto", ref)
             r["vs_fp32_dropin"] = round(r["value"] / value, 3)
             extras["mixed"][prec] = r
@@ -348,7 +349,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision == "fp32" else "f32 fields, %s coefficients" % args.precision,
+        "dtype": "f32" if args.precision in ("fp32", "fp32r") else "f32 fields, %s coefficients" % args.precision,
         "data": "synthetic seeded heterogeneous TTI model (vpz gradient + blobs, smooth eps/delta/theta/phi, "
                 "reference random boundary)",
         "config": {"workload": "TTI %d^3 interior, absorb %d, array %dx%dx%d, dx=12.5 m, dt=%.6g s, "

@@ -135,8 +135,8 @@ static int dev_alloc(void **p, size_t bytes, long long *acct)
     return FL_OK;
 }
 
-static int coef_count(int cm) { return cm == CM_F16R ? 6 : 10; }
-static int coef_elem(int cm) { return cm == CM_F32 ? 4 : 2; }
+static int coef_count(int cm) { return (cm == CM_F16R || cm == CM_F32R) ? 6 : 10; }
+static int coef_elem(int cm) { return (cm == CM_F32 || cm == CM_F32R) ? 4 : 2; }
 
 extern "C" int fl_bytes_per_cell(void)
 {
@@ -147,7 +147,8 @@ extern "C" int fl_bytes_per_cell(void)
 }
 
 // Host precompute into the device coefficient records: per cell, its coef_count(cm)
-// values in precomp.h order (FP32) or the 16-bit forms, consecutive (record of
+// values (FP32 or the 16-bit forms, or the recompute inputs) in the slot order
+// rec_order32 / rec_order16 / rec_order_r32 / rec_order_r16 (record of
 // coef_count * coef_elem bytes); z slabs of at most 64 MiB per array on the host.
 static int upload_coefs(const float *vpz, const float *vsv, const float *eps, const float *del,
                         const float *phi, const float *theta, float dt, int cm, void *dst, int sx,
@@ -163,13 +164,18 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
     for (int z0 = 0; z0 < sz; z0 += slab) {
         const int z1 = std::min(sz, z0 + slab);
         const long long off = (long long)z0 * sxy, m = (long long)(z1 - z0) * sxy;
-        if (cm == CM_F32) {
-            f32.resize(10 * m);
-            r32.resize(10 * m);
+        if (cm == CM_F32 || cm == CM_F32R) {
+            f32.resize((size_t)nk * m);
+            r32.resize((size_t)nk * m);
             float *c[10];
-            for (int k = 0; k < 10; k++) c[k] = f32.data() + k * m;
-            fl_precompute_f32(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, c);
-            fl_interleave32(m, 10, c, r32.data());
+            for (int k = 0; k < nk; k++) c[k] = f32.data() + k * m;
+            if (cm == CM_F32)
+                fl_precompute_f32(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, c);
+            else
+                fl_pack_recompute32(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, c);
+            const float *co[10];
+            for (int k = 0; k < nk; k++) co[k] = c[cm == CM_F32 ? rec_order32(k) : rec_order_r32(k)];
+            fl_interleave32(m, nk, co, r32.data());
             HIPCHK(hipMemcpy2D((char *)dst + (size_t)z0 * sy * px * rec, px * rec, r32.data(), sx * rec,
                                sx * rec, (size_t)sy * (z1 - z0), hipMemcpyHostToDevice));
         } else {
@@ -182,7 +188,9 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
             else
                 fl_pack_coefs16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
                                 cm == CM_BF16, c);
-            fl_interleave16(m, nk, c, r16.data());
+            const uint16_t *co[10];
+            for (int k = 0; k < nk; k++) co[k] = c[cm == CM_F16R ? rec_order_r16(k) : rec_order16(k)];
+            fl_interleave16(m, nk, co, r16.data());
             HIPCHK(hipMemcpy2D((char *)dst + (size_t)z0 * sy * px * rec, px * rec, r16.data(), sx * rec,
                                sx * rec, (size_t)sy * (z1 - z0), hipMemcpyHostToDevice));
         }
@@ -230,6 +238,7 @@ static int launch_steps(const StepArgs &a, int ty, int cm, bool tp, hipStream_t 
         case CM_BF16: launch_zm<TY, CM_BF16, false>(a, s); return FL_OK;   \
         case CM_F16: launch_zm<TY, CM_F16, false>(a, s); return FL_OK;     \
         case CM_F16R: launch_zm<TY, CM_F16R, false>(a, s); return FL_OK;   \
+        case CM_F32R: launch_zm<TY, CM_F32R, false>(a, s); return FL_OK;   \
         }                                                                  \
     }
     FL_ZM(4)
@@ -434,7 +443,7 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
 
 // Selection through the environment for front-ends that only call driver.h (the
 // reference's main.c/model.c): FL_VARIANT = fused|direct|twopass, FL_PRECISION =
-// fp32|bf16|fp16|fp16r, applied at DRIVER_Initialize when set; FL_TUNE = cache file, the
+// fp32|bf16|fp16|fp16r|fp32r, applied at DRIVER_Initialize when set; FL_TUNE = cache file, the
 // autotuner runs (or reads the cache) right after it.
 static int env_pick(const char *name, const char *const *opts, int n)
 {
@@ -454,8 +463,8 @@ extern "C" void DRIVER_Initialize(const int sx, const int sy, const int sz, cons
                                   float *__restrict__ qc)
 {
     static const char *const variants[] = {"fused", "direct", "twopass"};
-    static const char *const precs[] = {"fp32", "bf16", "fp16", "fp16r"};
-    const int v = env_pick("FL_VARIANT", variants, 3), p = env_pick("FL_PRECISION", precs, 4);
+    static const char *const precs[] = {"fp32", "bf16", "fp16", "fp16r", "fp32r"};
+    const int v = env_pick("FL_VARIANT", variants, 3), p = env_pick("FL_PRECISION", precs, 5);
     if (v == -2 || p == -2) {
         fail(FL_ERR_ARG, "FL_VARIANT / FL_PRECISION hold an unknown value");
         return;
@@ -546,7 +555,7 @@ extern "C" int fl_get_variant(void) { return g_variant; }
 
 extern "C" int fl_set_precision(int p)
 {
-    if (p < FL_PREC_FP32 || p > FL_PREC_FP16_RECOMPUTE) return fail(FL_ERR_ARG, "unknown precision %d", p);
+    if (p < FL_PREC_FP32 || p > FL_PREC_FP32_RECOMPUTE) return fail(FL_ERR_ARG, "unknown precision %d", p);
     g_precision = p;
     return FL_OK;
 }

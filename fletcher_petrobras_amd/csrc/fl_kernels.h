@@ -49,8 +49,47 @@ enum CoefMode : int {
     CM_F32 = 0,   // 10 FP32 volumes, precomp.h layout
     CM_BF16 = 1,  // 10 BF16 volumes; v2* pre-multiplied by dt^2
     CM_F16 = 2,   // 10 FP16 volumes; v2* pre-multiplied by dt^2
-    CM_F16R = 3,  // 6 FP16 volumes (vpz*dt, vsv*dt, eps, delta, theta, phi-pi), recomputed per cell
+    CM_F16R = 3,  // 4 FP16 velocity factors (times dt^2) and theta, phi as u16 turns
+    CM_F32R = 4,  // 4 FP32 velocity factors (precomp.h values) and theta, phi as FP32 turns
 };
+
+// Bytes of one cell's coefficient record.
+__host__ __device__ constexpr int rec_bytes(int cm)
+{
+    return cm == CM_F32 ? 40 : cm == CM_F16R ? 12 : cm == CM_F32R ? 24 : 20;
+}
+
+// Order of the precomp.h coefficients (0 ch1dxx, 1 ch1dyy, 2 ch1dzz, 3 ch1dxy, 4 ch1dyz,
+// 5 ch1dxz, 6 v2px, 7 v2pz, 8 v2sz, 9 v2pn) within a stored record: slot s holds
+// coefficient REC_ORDER[s].  The step kernel has the two lanes of a cell (p and q) load
+// one half of the record each and broadcast it to the pair:
+//   FP32 (40 B):        p lane {0 1 2 3 6}, q lane {4 5 7 8 9}; the fifth value is the
+//                       lane's own v2px / v2pn.
+//   BF16 / FP16 (20 B): p lane {0 1 2 3}, q lane {4 5 7 8}, both {6 9}.
+// The recompute inputs are v2px, v2pz, v2sz, v2pn, theta, phi (0..5), stored as
+//   FP16 recompute (12 B): words (v2px, v2pn), (v2pz, v2sz), (theta, phi), loaded whole
+//                          by both lanes;
+//   FP32 recompute (24 B): p lane {v2px, v2pz, theta}, q lane {v2pn, v2sz, phi}.
+__host__ __device__ constexpr int rec_order32(int s)
+{
+    constexpr int o[10] = {0, 1, 2, 3, 6, 4, 5, 7, 8, 9};
+    return o[s];
+}
+__host__ __device__ constexpr int rec_order16(int s)
+{
+    constexpr int o[10] = {0, 1, 2, 3, 4, 5, 7, 8, 6, 9};
+    return o[s];
+}
+__host__ __device__ constexpr int rec_order_r16(int s)
+{
+    constexpr int o[6] = {0, 3, 1, 2, 4, 5};
+    return o[s];
+}
+__host__ __device__ constexpr int rec_order_r32(int s)
+{
+    constexpr int o[6] = {0, 1, 4, 3, 2, 5};
+    return o[s];
+}
 
 struct StepArgs {
     float *prev;          // level read at the own cell and overwritten (pp, qp)
@@ -267,7 +306,7 @@ __global__ __launch_bounds__(256) void direct_dercross(const DirectArgs a)
         const float qxz = dercross_g(qc, i, sX, sZ, a.dxzinv);
         Coef k;
 #pragma unroll
-        for (int j = 0; j < 10; j++) k.c[j] = a.coef[c * 10 + j];
+        for (int j = 0; j < 10; j++) k.c[rec_order32(j)] = a.coef[c * 10 + j];
         float ppv = a.prev[i], qpv = a.prev[i + 1];
         combine<CM_F32>(k, a.dt, pxx, pyy, pzz, pxy, pyz, pxz, qxx, qyy, qzz, qxy, qyz, qxz,
                         pc[i], qc[i], ppv, qpv);

@@ -79,65 +79,127 @@ __device__ __forceinline__ void bstore32(__amdgpu_buffer_rsrc_t r, uint32_t voff
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
 }
 
-// Coefficient record of one cell as loaded: FP32 10 values (40 B); BF16 / FP16 10 values
-// (20 B); FP16 recompute inputs (12 B: 4 FP16 velocity factors, 2 u16 angles).  Kept raw until used, so the loads stay
-// in flight across barriers.
+// Coefficient record as loaded by one lane: FP32 its half of the record (5 values, 20 B);
+// BF16 / FP16 its half (2 words) and the shared word of v2px, v2pn; FP32 recompute its
+// half (own velocity factor, v2pz or v2sz, own angle: 12 B); FP16 recompute the whole
+// record (4 FP16 velocity factors, 2 u16 angles).  Kept raw until used, so the
+// loads stay in flight across barriers.
 template <int CM>
 struct CoefRec {
-    static constexpr int NW = (CM == CM_F32) ? 10 : (CM == CM_F16R ? 3 : 5);
+    static constexpr int NW = (CM == CM_F32) ? 5 : 3;
     uint32_t w[NW];
 };
 
+// Lane byte offsets of a cell's record: the own half (voff) and, for the 16-bit forms,
+// the shared last word (voff2).
+struct RecOff {
+    uint32_t voff, voff2;
+};
 template <int CM>
-__device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, uint32_t voff, CoefRec<CM> &c)
+__device__ __forceinline__ RecOff rec_offsets(uint32_t col, int fld, bool active)
 {
+    RecOff o;
     if constexpr (CM == CM_F32) {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
-        const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 16, 0);
-        const auto d = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 32, 0);
-        c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
-        c.w[4] = b[0]; c.w[5] = b[1]; c.w[6] = b[2]; c.w[7] = b[3];
-        c.w[8] = d[0]; c.w[9] = d[1];
+        o.voff = active ? col * 40u + (uint32_t)fld * 20u : FL_OOB;
+        o.voff2 = 0;
     } else if constexpr (CM == CM_F16R) {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b96(r, voff, 0, 0);
-        c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2];
+        o.voff = active ? col * 12u : FL_OOB;
+        o.voff2 = 0;
+    } else if constexpr (CM == CM_F32R) {
+        o.voff = active ? col * 24u + (uint32_t)fld * 12u : FL_OOB;
+        o.voff2 = 0;
     } else {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
-        c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
-        c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, voff, 16, 0);
+        o.voff = active ? col * 20u + (uint32_t)fld * 8u : FL_OOB;
+        o.voff2 = active ? col * 20u + 16u : FL_OOB;
     }
+    return o;
 }
 
 template <int CM>
-__device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, Coef &k)
+__device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, const RecOff &o, CoefRec<CM> &c)
 {
     if constexpr (CM == CM_F32) {
-#pragma unroll
-        for (int j = 0; j < 10; j++) k.c[j] = __builtin_bit_cast(float, r.w[j]);
-    } else if constexpr (CM == CM_BF16) {
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-            k.c[2 * j] = __builtin_bit_cast(float, r.w[j] << 16);
-            k.c[2 * j + 1] = __builtin_bit_cast(float, r.w[j] & 0xffff0000u);
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, o.voff, 0, 0);
+        c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
+        c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff, 16, 0);
+    } else if constexpr (CM == CM_F16R || CM == CM_F32R) {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b96(r, o.voff, 0, 0);
+        c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2];
+    } else {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b64(r, o.voff, 0, 0);
+        c.w[0] = a[0]; c.w[1] = a[1];
+        c.w[2] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff2, 0, 0);
+    }
+}
+
+// Value of the p lane (even) or of the q lane (odd) of each lane pair, in both lanes
+// (DPP quad_perm [0,0,2,2] and [1,1,3,3]).
+__device__ __forceinline__ uint32_t from_p(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xA0, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t from_q(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xF5, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
+
+// The coefficients of the own cell in precomp.h order.  For the stored forms c[6] and c[9]
+// both hold the lane's own factor (v2px on p lanes, v2pn on q lanes), the only one the
+// update of the lane's field reads.
+template <int CM>
+__device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, Coef &k)
+{
+    if constexpr (CM == CM_F32) {
+        k.c[0] = __builtin_bit_cast(float, from_p(r.w[0]));
+        k.c[1] = __builtin_bit_cast(float, from_p(r.w[1]));
+        k.c[2] = __builtin_bit_cast(float, from_p(r.w[2]));
+        k.c[3] = __builtin_bit_cast(float, from_p(r.w[3]));
+        k.c[4] = __builtin_bit_cast(float, from_q(r.w[0]));
+        k.c[5] = __builtin_bit_cast(float, from_q(r.w[1]));
+        k.c[7] = __builtin_bit_cast(float, from_q(r.w[2]));
+        k.c[8] = __builtin_bit_cast(float, from_q(r.w[3]));
+        k.c[6] = k.c[9] = __builtin_bit_cast(float, r.w[4]);
+    } else if constexpr (CM == CM_BF16 || CM == CM_F16) {
+        const uint32_t a0 = from_p(r.w[0]), a1 = from_p(r.w[1]);
+        const uint32_t b0 = from_q(r.w[0]), b1 = from_q(r.w[1]);
+        const uint32_t own = r.w[2] >> (16 * fld);
+        if constexpr (CM == CM_BF16) {
+            k.c[0] = bf16_lo(a0); k.c[1] = bf16_hi(a0);
+            k.c[2] = bf16_lo(a1); k.c[3] = bf16_hi(a1);
+            k.c[4] = bf16_lo(b0); k.c[5] = bf16_hi(b0);
+            k.c[7] = bf16_lo(b1); k.c[8] = bf16_hi(b1);
+            k.c[6] = k.c[9] = bf16_lo(own);
+        } else {
+            k.c[0] = f16_to_f32((uint16_t)a0); k.c[1] = f16_to_f32((uint16_t)(a0 >> 16));
+            k.c[2] = f16_to_f32((uint16_t)a1); k.c[3] = f16_to_f32((uint16_t)(a1 >> 16));
+            k.c[4] = f16_to_f32((uint16_t)b0); k.c[5] = f16_to_f32((uint16_t)(b0 >> 16));
+            k.c[7] = f16_to_f32((uint16_t)b1); k.c[8] = f16_to_f32((uint16_t)(b1 >> 16));
+            k.c[6] = k.c[9] = f16_to_f32((uint16_t)own);
         }
-    } else if constexpr (CM == CM_F16) {
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-            k.c[2 * j] = f16_to_f32((uint16_t)r.w[j]);
-            k.c[2 * j + 1] = f16_to_f32((uint16_t)(r.w[j] >> 16));
+    } else {
+        // recomputed forms: the lane's own velocity factor and angle (p lanes v2px and
+        // theta, q lanes v2pn and phi), v2pz and v2sz from the pair
+        float own, vpz, vsz, t;
+        if constexpr (CM == CM_F16R) {  // words (v2px, v2pn), (v2pz, v2sz), (theta, phi)
+            own = f16_to_f32((uint16_t)(r.w[0] >> (16 * fld)));
+            vpz = f16_to_f32((uint16_t)r.w[1]);
+            vsz = f16_to_f32((uint16_t)(r.w[1] >> 16));
+            t = (float)((r.w[2] >> (16 * fld)) & 0xffffu) * (1.0f / 65536.0f);
+        } else {  // CM_F32R: lane words (own factor, v2pz | v2sz, theta | phi turns)
+            own = __builtin_bit_cast(float, r.w[0]);
+            vpz = __builtin_bit_cast(float, from_p(r.w[1]));
+            vsz = __builtin_bit_cast(float, from_q(r.w[1]));
+            t = __builtin_bit_cast(float, r.w[2]);
         }
-    } else {  // CM_F16R: v2px, v2pz, v2sz, v2pn (times dt^2, FP16), theta, phi (u16 turns)
-        float v[4];
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            v[2 * j] = f16_to_f32((uint16_t)r.w[j]);
-            v[2 * j + 1] = f16_to_f32((uint16_t)(r.w[j] >> 16));
-        }
-        // the angular coefficients of precomp.h:25-38; v_sin/v_cos take turns
-        const float tt = (float)(r.w[2] & 0xffffu) * (1.0f / 65536.0f);
-        const float tp = (float)(r.w[2] >> 16) * (1.0f / 65536.0f);
-        const float st = __builtin_amdgcn_sinf(tt), ct = __builtin_amdgcn_cosf(tt);
-        const float sp = __builtin_amdgcn_sinf(tp), cp = __builtin_amdgcn_cosf(tp);
+        // the angular coefficients of precomp.h:25-38; v_sin / v_cos take turns, each
+        // lane evaluates its own angle and the pair exchanges them
+        const float sn = __builtin_amdgcn_sinf(t), cs = __builtin_amdgcn_cosf(t);
+        const float st = __builtin_bit_cast(float, from_p(__builtin_bit_cast(uint32_t, sn)));
+        const float ct = __builtin_bit_cast(float, from_p(__builtin_bit_cast(uint32_t, cs)));
+        const float sp = __builtin_bit_cast(float, from_q(__builtin_bit_cast(uint32_t, sn)));
+        const float cp = __builtin_bit_cast(float, from_q(__builtin_bit_cast(uint32_t, cs)));
         const float st2 = st * st;
         const float s2t = 2.0f * st * ct;
         const float s2p = 2.0f * sp * cp;
@@ -147,10 +209,9 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, Coef &k)
         k.c[3] = st2 * s2p;
         k.c[4] = s2t * sp;
         k.c[5] = s2t * cp;
-        k.c[6] = v[0];
-        k.c[7] = v[1];
-        k.c[8] = v[2];
-        k.c[9] = v[3];
+        k.c[6] = k.c[9] = own;
+        k.c[7] = vpz;
+        k.c[8] = vsz;
     }
 }
 
@@ -184,7 +245,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     constexpr int NS = RW * HY;                // floats per staged plane (a multiple of 64)
     constexpr int NLD = (NS + NT - 1) / NT;    // staging loads per lane
     constexpr int XW2 = 2 * TX;                // floats per Dx row
-    constexpr int REC = (CM == CM_F32) ? 40 : (CM == CM_F16R ? 12 : 20);  // record bytes
+    constexpr int REC = rec_bytes(CM);
     constexpr int NH = (8 + TY - 1) / TY;      // y-halo rows of Dx per wave (at most)
     static_assert(NS % 64 == 0, "staged plane must be whole waves");
     const int x0 = tix * TX, y0 = a.bord + tiy * TY;
@@ -202,7 +263,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     const uint32_t pb8 = pxy * 8u, pbr = pxy * (uint32_t)REC;
     const uint32_t col = (uint32_t)(y * a.px + x);
     const uint32_t vo8 = active ? col * 8u + fld * 4u : FL_OOB;  // own (cell, field) float
-    const uint32_t vor = active ? col * (uint32_t)REC : FL_OOB;
+    const RecOff vor = rec_offsets<CM>(col, fld, active);
 
     const long long zin = (long long)f0 * pxy, zout = (long long)zlo * pxy;
     const __amdgpu_buffer_rsrc_t r_cur = span_rsrc(a.cur + 2 * zin, (uint32_t)(nz + 8) * pb8);
@@ -389,7 +450,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             const float dxz = der1_q(qzx, a.dzinv);
             const float dyz = der1_q(qzy, a.dzinv);
             Coef kc;
-            decode_rec<CM>(kr, kc);
+            decode_rec<CM>(kr, fld, kc);
             // H1, H2 of the own field, association order of original/sample.h:46-53
             float h1 = kc.c[0] * dxx;
             h1 = __builtin_fmaf(kc.c[1], dyy, h1);
@@ -412,7 +473,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             const float cur = qz[4];
             const float base = 2.0f * cur - vprev;
             float nw;
-            if constexpr (CM == CM_F32) nw = __builtin_fmaf(rhs * a.dt, a.dt, base);
+            if constexpr (CM == CM_F32 || CM == CM_F32R) nw = __builtin_fmaf(rhs * a.dt, a.dt, base);
             else nw = base + rhs;
             if (z == rec_z) {
                 if (fld == 0 && active)
@@ -481,12 +542,18 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
 
 // Waves per SIMD the register allocator is asked for.  With 8-row tiles the LDS rings
 // (51 KB per workgroup) allow three workgroups per CU: 6 waves per SIMD, <= 80 VGPRs,
-// which the 16-bit coefficient forms meet; FP32 records keep two (4 waves, <= 128).
-// 4- and 16-row tiles are held to 4 waves per SIMD by their LDS.
+// which the 16-bit and recompute forms meet.  The FP32 records keep two workgroups
+// (4 waves, 85 VGPRs): held to 80 they spill one value per step and run 7% slower at
+// 344^3.  4- and 16-row tiles are held to 4 waves per SIMD by their LDS.
+// FL_STEP_WAVES overrides the 8-row value at build time.
 template <int TY, int CM>
 constexpr int step_waves()
 {
+#ifdef FL_STEP_WAVES
+    return TY == 8 ? FL_STEP_WAVES : 4;
+#else
     return (TY == 8 && CM != CM_F32) ? 6 : 4;
+#endif
 }
 
 template <int TY, int CM, bool TWOPASS>
@@ -545,7 +612,7 @@ constexpr int batch_waves()
 #ifdef FL_BATCH_W4
     return 4;
 #else
-    return (TY == 8 && CM == CM_F16R) ? 6 : 4;
+    return (TY == 8 && (CM == CM_F16R || CM == CM_F32R)) ? 6 : 4;
 #endif
 }
 template <int TY, int CM>

@@ -314,6 +314,25 @@ void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const fl
     }
 }
 
+void fl_pack_recompute32(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                         const float *delta, const float *phi, const float *theta, float *const *out)
+{
+    const double pi = 3.14159265358979323846;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        /* the FP32 velocity factors of precomp.h:53-58, as fl_precompute_f32 forms them */
+        const float v2pz = vpz[i] * vpz[i];
+        out[0][i] = v2pz * (1.0 + 2.0 * epsilon[i]);
+        out[1][i] = v2pz;
+        out[2][i] = vsv[i] * vsv[i];
+        out[3][i] = v2pz * (1.0 + 2.0 * delta[i]);
+        /* angles in turns, reduced to [-1/2, 1/2] in FP64 and rounded once */
+        const double tt = (double)theta[i] / (2.0 * pi), tp = (double)phi[i] / (2.0 * pi);
+        out[4][i] = (float)(tt - nearbyint(tt));
+        out[5][i] = (float)(tp - nearbyint(tp));
+    }
+}
+
 void fl_interleave32(int64_t n, int nk, const float *const *src, float *dst)
 {
 #pragma omp parallel for schedule(static)

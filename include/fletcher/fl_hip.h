@@ -32,15 +32,24 @@ enum { FL_VARIANT_FUSED = 0, FL_VARIANT_DIRECT = 1, FL_VARIANT_TWOPASS = 2 };
  * store the same 10 volumes (v2* pre-multiplied by dt^2, evaluated in FP64 and
  * rounded once); FP16_RECOMPUTE stores the 4 velocity factors (times dt^2) in FP16
  * and theta, phi as 16-bit fixed point turns, and rebuilds the 6 angular coefficients
- * per cell.  Wavefields stay FP32. */
-enum { FL_PREC_FP32 = 0, FL_PREC_BF16 = 1, FL_PREC_FP16 = 2, FL_PREC_FP16_RECOMPUTE = 3 };
+ * per cell; FP32_RECOMPUTE stores the 4 precomp.h velocity factors bit for bit and
+ * theta, phi as FP32 turns, and rebuilds the angular coefficients in FP32 (24 bytes per
+ * cell instead of 40).  Wavefields stay FP32.  The recompute forms and the 16-bit forms
+ * run with the fused variant. */
+enum {
+    FL_PREC_FP32 = 0,
+    FL_PREC_BF16 = 1,
+    FL_PREC_FP16 = 2,
+    FL_PREC_FP16_RECOMPUTE = 3,
+    FL_PREC_FP32_RECOMPUTE = 4
+};
 
 const char *fl_last_error(void);
 int fl_status(void);      /* code of the last failed call, FL_OK after fl_clear_error */
 void fl_clear_error(void);
 
 /* Front-ends that call only driver.h select with the environment instead, read by
- * DRIVER_Initialize: FL_VARIANT=fused|direct|twopass, FL_PRECISION=fp32|bf16|fp16|fp16r,
+ * DRIVER_Initialize: FL_VARIANT=fused|direct|twopass, FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r,
  * FL_TUNE=<cache file> (autotune right after initialisation). */
 int fl_set_variant(int variant);   /* takes effect at the next step */
 int fl_get_variant(void);

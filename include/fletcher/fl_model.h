@@ -69,6 +69,12 @@ void fl_pack_coefs16(int64_t n, const float *vpz, const float *vsv, const float 
 void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, double dt,
                          uint16_t *const *out);
+/* Inputs of the in-kernel FP32 recompute: v2px, v2pz, v2sz, v2pn exactly as
+ * fl_precompute_f32 (precomp.h:53-58) forms them, and theta, phi in turns reduced to
+ * [-1/2, 1/2] (FP64, rounded once to FP32); the six angular coefficients are rebuilt
+ * from the angles per cell in FP32. */
+void fl_pack_recompute32(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                         const float *delta, const float *phi, const float *theta, float *const *out);
 /* Cell records: dst[i*nk + k] = src[k][i] (the device keeps the coefficients of a cell
  * together, one record per cell). */
 void fl_interleave32(int64_t n, int nk, const float *const *src, float *dst);

@@ -33,3 +33,14 @@ def test_reference_front_end_drives_the_device(oracle, cpu_ref, variant, monkeyp
     err = rel_l2(got["pc"], cpu_ref["pc"])
     print("%s: relL2(pc) vs reference OpenMP = %.3e" % (variant, err))
     assert err <= 1e-5
+
+
+def test_reference_front_end_fp32_recompute(oracle, cpu_ref, monkeypatch):
+    """FL_PRECISION=fp32r: 24-byte records, angular coefficients rebuilt in FP32 on the
+    device; the same 1e-5 bar against the reference OpenMP build."""
+    monkeypatch.setenv("FL_VARIANT", "fused")
+    monkeypatch.setenv("FL_PRECISION", "fp32r")
+    got = oracle.Ref("dropin").main(ARGS)
+    err = rel_l2(got["pc"], cpu_ref["pc"])
+    print("fp32r: relL2(pc) vs reference OpenMP = %.3e" % err)
+    assert err <= 1e-5

@@ -61,6 +61,21 @@ def test_reference_fixture_parity(fl, drv, form, variant, steps):
     assert rel_l2(d.field("pp"), g["pp"]) <= TOL
 
 
+@pytest.mark.parametrize("form", ["ISO", "VTI", "TTI"])
+@pytest.mark.parametrize("steps", [10, 30])
+def test_fp32_recompute_fixture_parity(fl, drv, form, steps):
+    """FP32 recompute (velocity factors of precomp.h bit for bit, the six angular
+    coefficients rebuilt in FP32 from theta, phi per cell) against the reference fixtures."""
+    g = golden("ref_%s_der1der1_n8_a4_s%d.npz" % (form, steps))
+    shape, m = ref_model(fl, form, 8, 4)
+    d = drv(variant="fused", precision="fp32r")
+    d.initialize(shape, D, 0.001, m)
+    d.step_loop(steps, int(g["isource"]))
+    assert rel_l2(d.update_pointers(), g["pc"]) <= TOL
+    assert rel_l2(d.field("qc"), g["qc"]) <= TOL
+    assert rel_l2(d.field("pp"), g["pp"]) <= TOL
+
+
 def test_reference_fixture_parity_absorb16(fl, drv):
     g = golden("ref_TTI_original_n24_a16_s10.npz")
     shape, m = ref_model(fl, "TTI", 24, 16)
@@ -95,6 +110,21 @@ def test_oracle_parity_heterogeneous_ragged(fl, oracle, drv, variant):
     r2 = oracle.run("der1der1" if ov == "original" else "original", shape, 4, D, dt, st, isrc,
                     oracle.precompute(m))
     assert rel_l2(d.field("pc"), r2["pc"]) <= TOL
+
+
+def test_fp32_recompute_oracle_parity_heterogeneous_ragged(fl, oracle, drv):
+    m, dt = hetero(fl)
+    shape = m[0].shape
+    isrc = fl.index(shape, shape[2] // 3, shape[1] // 2, 4 + 8 + 5)
+    rec, st = 4 + 8 + 5, 160
+    d = drv(variant="fused", precision="fp32r")
+    d.initialize(shape, D, dt, m)
+    d.record_traces(rec, 4, st // 4)
+    d.run(st, isrc, graph_chunk=20)
+    r = oracle.run("der1der1", shape, 4, D, dt, st, isrc, oracle.precompute(m), rec_iz=rec, rec_every=4)
+    e = [rel_l2(d.field("pc"), r["pc"]), rel_l2(d.field("qc"), r["qc"]), rel_l2(d.traces(), r["traces"])]
+    print("fp32r vs FP32 oracle relL2 pc %.3e qc %.3e traces %.3e" % tuple(e))
+    assert max(e) <= TOL
 
 
 def test_fp64_oracle_error_reported(fl, oracle, drv):

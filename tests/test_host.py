@@ -217,6 +217,33 @@ def test_precompute_f64_equals_oracle_f64(fl, oracle):
         assert np.array_equal(x, y), fl.COEF_NAMES[k]
 
 
+def test_pack_recompute32_inputs(fl):
+    """FP32 recompute inputs: the four velocity factors bit for bit those of the FP32
+    precompute (precomp.h:53-58), and theta, phi as FP32 turns in [-1/2, 1/2] whose
+    sine and cosine give the reference's angles to FP32 rounding."""
+    m = fl.model_gradient((16, 14, 12), 4, seed=11)
+    n = m[0].size
+    c32 = fl.precompute(m, "f32")
+    out = [np.empty(m[0].shape, np.float32) for _ in range(6)]
+    fl.lib().fl_pack_recompute32(n, *m, (C.c_void_p * 6)(*[o.ctypes.data for o in out]))
+    for k, j in enumerate((6, 7, 8, 9)):
+        assert np.array_equal(out[k], c32[j]), k
+    for k, ang in ((4, m[5]), (5, m[4])):  # theta, phi
+        t = out[k].astype(np.float64)
+        assert np.all(np.abs(t) <= 0.5)
+        a = ang.astype(np.float64)
+        assert np.abs(np.sin(2 * np.pi * t) - np.sin(a)).max() < 4e-7
+        assert np.abs(np.cos(2 * np.pi * t) - np.cos(a)).max() < 4e-7
+    # the angular coefficients rebuilt from them in FP64 agree with the FP64 precompute
+    c64 = fl.precompute(m, "f64")
+    st, ct = np.sin(2 * np.pi * out[4].astype(np.float64)), np.cos(2 * np.pi * out[4].astype(np.float64))
+    sp, cp = np.sin(2 * np.pi * out[5].astype(np.float64)), np.cos(2 * np.pi * out[5].astype(np.float64))
+    rebuilt = [st * st * cp * cp, st * st * sp * sp, ct * ct, st * st * 2 * sp * cp, 2 * st * ct * sp,
+               2 * st * ct * cp]
+    for j in range(6):
+        assert np.abs(rebuilt[j] - c64[j]).max() < 1e-6, j
+
+
 def test_pack_recompute16_inputs(fl):
     """FP16 recompute inputs: the FP64 velocity factors times dt^2, rounded once (numpy's
     correctly rounded float64 -> float16), and the angles as u16 turns."""

@@ -221,30 +221,40 @@ static void fill_geom(StepArgs &a, int sx, int sy, int sz, int bord, int ty, int
     a.nblk = a.ntx * a.nty * nzc;
 }
 
-template <int TY, int CM, bool TP>
+template <int TY, int CM, bool TP, bool ISO>
 static void launch_zm(const StepArgs &a, hipStream_t s)
 {
     const int g = (a.nblk + 7) / 8 * 8;
-    hipLaunchKernelGGL((tti_step<TY, CM, TP>), dim3(g), dim3(64, TY), 0, s, a);
+    hipLaunchKernelGGL((tti_step<TY, CM, TP, ISO>), dim3(g), dim3(64, TY), 0, s, a);
 }
+
+// Equal spacings select the kernels that scale the right-hand side once (ISO).
+static bool iso_spacing(float dx, float dy, float dz) { return dx == dy && dy == dz; }
 
 static int launch_steps(const StepArgs &a, int ty, int cm, bool tp, hipStream_t s)
 {
+    const bool iso = iso_spacing(D.dx, D.dy, D.dz);
+#define FL_ZM1(TY, CM)                                                     \
+    case CM:                                                               \
+        if (iso) launch_zm<TY, CM, false, true>(a, s);                     \
+        else launch_zm<TY, CM, false, false>(a, s);                        \
+        return FL_OK;
 #define FL_ZM(TY)                                                          \
     if (ty == TY) {                                                        \
-        if (tp) { launch_zm<TY, CM_F32, true>(a, s); return FL_OK; }       \
+        if (tp) { launch_zm<TY, CM_F32, true, false>(a, s); return FL_OK; } \
         switch (cm) {                                                      \
-        case CM_F32: launch_zm<TY, CM_F32, false>(a, s); return FL_OK;     \
-        case CM_BF16: launch_zm<TY, CM_BF16, false>(a, s); return FL_OK;   \
-        case CM_F16: launch_zm<TY, CM_F16, false>(a, s); return FL_OK;     \
-        case CM_F16R: launch_zm<TY, CM_F16R, false>(a, s); return FL_OK;   \
-        case CM_F32R: launch_zm<TY, CM_F32R, false>(a, s); return FL_OK;   \
+            FL_ZM1(TY, CM_F32)                                             \
+            FL_ZM1(TY, CM_BF16)                                            \
+            FL_ZM1(TY, CM_F16)                                             \
+            FL_ZM1(TY, CM_F16R)                                            \
+            FL_ZM1(TY, CM_F32R)                                            \
         }                                                                  \
     }
     FL_ZM(4)
     FL_ZM(8)
     FL_ZM(16)
 #undef FL_ZM
+#undef FL_ZM1
     return fail(FL_ERR_ARG, "unsupported tile rows %d / coefficient mode %d", ty, cm);
 }
 

@@ -144,14 +144,18 @@ __device__ __forceinline__ StepCtl step_ctl(const StepArgs &a)
 // so every copy of an expression rounds identically (results do not depend on tile or
 // chunk shapes).  The sums keep the reference association order (derivatives.h:47,53):
 // ((w1 d1 + w2 d2) + w3 d3) + w4 d4, each step one FMA.
+// SC = false leaves the sum unscaled (dinv unused): the caller applies the grid spacing
+// once to the combination instead.
+template <bool SC = true>
 __device__ __forceinline__ float fd1(float d1, float d2, float d3, float d4, float dinv)
 {
     float t = FL_L1 * d1;
     t = __builtin_fmaf(FL_L2, d2, t);
     t = __builtin_fmaf(FL_L3, d3, t);
     t = __builtin_fmaf(FL_L4, d4, t);
-    return t * dinv;
+    return SC ? t * dinv : t;
 }
+template <bool SC = true>
 __device__ __forceinline__ float fd2(float c, float s1, float s2, float s3, float s4, float d2inv)
 {
     float t = FL_K0 * c;
@@ -159,29 +163,33 @@ __device__ __forceinline__ float fd2(float c, float s1, float s2, float s3, floa
     t = __builtin_fmaf(FL_K2, s2, t);
     t = __builtin_fmaf(FL_K3, s3, t);
     t = __builtin_fmaf(FL_K4, s4, t);
-    return t * d2inv;
+    return SC ? t * d2inv : t;
 }
 
+template <bool SC = true>
 __device__ __forceinline__ float der1_lds(const float *p, int s, float dinv)
 {
-    return fd1(p[s] - p[-s], p[2 * s] - p[-2 * s], p[3 * s] - p[-3 * s], p[4 * s] - p[-4 * s], dinv);
+    return fd1<SC>(p[s] - p[-s], p[2 * s] - p[-2 * s], p[3 * s] - p[-3 * s], p[4 * s] - p[-4 * s], dinv);
 }
 
+template <bool SC = true>
 __device__ __forceinline__ float der2_lds(const float *p, int s, float d2inv)
 {
-    return fd2(p[0], p[s] + p[-s], p[2 * s] + p[-2 * s], p[3 * s] + p[-3 * s], p[4 * s] + p[-4 * s],
+    return fd2<SC>(p[0], p[s] + p[-s], p[2 * s] + p[-2 * s], p[3 * s] + p[-3 * s], p[4 * s] + p[-4 * s],
                d2inv);
 }
 
 // q[0..8] = values at z-4..z+4
+template <bool SC = true>
 __device__ __forceinline__ float der1_q(const float *q, float dinv)
 {
-    return fd1(q[5] - q[3], q[6] - q[2], q[7] - q[1], q[8] - q[0], dinv);
+    return fd1<SC>(q[5] - q[3], q[6] - q[2], q[7] - q[1], q[8] - q[0], dinv);
 }
 
+template <bool SC = true>
 __device__ __forceinline__ float der2_q(const float *q, float d2inv)
 {
-    return fd2(q[4], q[5] + q[3], q[6] + q[2], q[7] + q[1], q[8] + q[0], d2inv);
+    return fd2<SC>(q[4], q[5] + q[3], q[6] + q[2], q[7] + q[1], q[8] + q[0], d2inv);
 }
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }

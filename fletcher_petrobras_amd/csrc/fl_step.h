@@ -136,11 +136,11 @@ __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, const RecOff 
 // (DPP quad_perm [0,0,2,2] and [1,1,3,3]).
 __device__ __forceinline__ uint32_t from_p(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xA0, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xA0, 0xF, 0xF, true);
 }
 __device__ __forceinline__ uint32_t from_q(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xF5, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xF5, 0xF, 0xF, true);
 }
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
@@ -236,7 +236,10 @@ __device__ __forceinline__ void unroll_phases(F &&fn, std::integer_sequence<int,
 
 // One tile of one z chunk.  Output planes z in [zlo, zhi) of the previous level are
 // overwritten once.
-template <int TY, int CM, bool TWOPASS>
+// ISO (dx = dy = dz = h): the derivatives stay unscaled sums and 1/h^2 multiplies the
+// right-hand side once, in the leapfrog (the same expression up to rounding: every term
+// of H1, H2 carries 1/h^2).
+template <int TY, int CM, bool TWOPASS, bool ISO>
 __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl, int tix, int tiy, int zc,
                                           StepLds<TY> &s)
 {
@@ -259,6 +262,8 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     const int fld = lane & 1, cx = lane >> 1;
     const int x = x0 + cx, y = y0 + wv;
     const bool active = (x >= a.bord) && (x < a.sx - a.bord) && (y < a.sy - a.bord);
+    static_assert(!(ISO && TWOPASS), "two-pass derivative volumes are scaled");
+    const float dts = ISO ? a.dxxinv * a.dt : a.dt;  // scale of rhs in the FP32 leapfrog
     const uint32_t pxy = (uint32_t)a.px * (uint32_t)a.sy;  // cells per pitched plane
     const uint32_t pb8 = pxy * 8u, pbr = pxy * (uint32_t)REC;
     const uint32_t col = (uint32_t)(y * a.px + x);
@@ -404,8 +409,8 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         const float *cp = Tf + ow + CW;
         float nc = cp[0];
         if constexpr (!TWOPASS) {
-            nx = der1_lds(cp, 2, a.dxinv);
-            ny = der1_lds(cp, RW, a.dyinv);
+            nx = der1_lds<!ISO>(cp, 2, a.dxinv);
+            ny = der1_lds<!ISO>(cp, RW, a.dyinv);
         }
         if (xw) {
             Xf[oxw + CX] = nx;
@@ -413,7 +418,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             for (int m = 0; m < NH; m++)
                 if (hrow[m]) {
                     if constexpr (TWOPASS) Xf[ohx[m]] = gxh[m];
-                    else Xf[ohx[m]] = der1_lds(Tf + ohr[m] + 8, 2, a.dxinv);
+                    else Xf[ohx[m]] = der1_lds<!ISO>(Tf + ohr[m] + 8, 2, a.dxinv);
                 }
         }
         // formed here, in this step: keeps the compiler from carrying the raw LDS values to
@@ -429,9 +434,9 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         float dxx = 0.f, dyy = 0.f, dxy = 0.f;
         if constexpr (OUT) {
             const float *zp = Tz + ow + CW;
-            dxx = der2_lds(zp, 2, a.dxxinv);
-            dyy = der2_lds(zp, RW, a.dyyinv);
-            dxy = der1_lds(Xz + oxw + CX, XW2, a.dyinv);
+            dxx = der2_lds<!ISO>(zp, 2, a.dxxinv);
+            dyy = der2_lds<!ISO>(zp, RW, a.dyyinv);
+            dxy = der1_lds<!ISO>(Xz + oxw + CX, XW2, a.dyinv);
         }
         // next plane into the ring from its set, then the set refilled SA planes ahead
         if (i + 1 <= nz + 7) stage(ic<(SET + 1) % SA>{}, s.t[r6n]);
@@ -446,9 +451,9 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
                 qzx[k] = qx_[(b + k) % 10];
                 qzy[k] = qy_[(b + k) % 10];
             }
-            const float dzz = der2_q(qz, a.dzzinv);
-            const float dxz = der1_q(qzx, a.dzinv);
-            const float dyz = der1_q(qzy, a.dzinv);
+            const float dzz = der2_q<!ISO>(qz, a.dzzinv);
+            const float dxz = der1_q<!ISO>(qzx, a.dzinv);
+            const float dyz = der1_q<!ISO>(qzy, a.dzinv);
             Coef kc;
             decode_rec<CM>(kr, fld, kc);
             // H1, H2 of the own field, association order of original/sample.h:46-53
@@ -462,7 +467,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             // p lanes receive h1q, q lanes h2p: swap within lane pairs (DPP quad_perm 1,0,3,2)
             const float send = fld ? h1 : h2;
             const float other = __builtin_bit_cast(
-                float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, send), 0xB1, 0xF, 0xF, false));
+                float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, send), 0xB1, 0xF, 0xF, true));
             // rhsp = v2px h2p + v2pz h1q + v2sz (h1p - h1q)
             // rhsq = v2pn h2p + v2pz h1q - v2sz (h2p - h2q) = ... + v2sz (h2q - h2p)
             // with h2p, h1q selected per half (original/sample.h:75-81, same roundings)
@@ -473,7 +478,8 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             const float cur = qz[4];
             const float base = 2.0f * cur - vprev;
             float nw;
-            if constexpr (CM == CM_F32 || CM == CM_F32R) nw = __builtin_fmaf(rhs * a.dt, a.dt, base);
+            if constexpr (CM == CM_F32 || CM == CM_F32R) nw = __builtin_fmaf(rhs * dts, a.dt, base);
+            else if constexpr (ISO) nw = __builtin_fmaf(rhs, a.dxxinv, base);
             else nw = base + rhs;
             if (z == rec_z) {
                 if (fld == 0 && active)
@@ -556,7 +562,7 @@ constexpr int step_waves()
 #endif
 }
 
-template <int TY, int CM, bool TWOPASS>
+template <int TY, int CM, bool TWOPASS, bool ISO>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(step_waves<TY, CM>())))
 void tti_step(const StepArgs a)
 {
@@ -568,7 +574,7 @@ void tti_step(const StepArgs a)
     if (t >= a.nblk) return;
     const int tix = t % a.ntx;
     const int rest = t / a.ntx;
-    step_tile<TY, CM, TWOPASS>(a, ctl, tix, rest % a.nty, rest / a.nty, s);
+    step_tile<TY, CM, TWOPASS, ISO>(a, ctl, tix, rest % a.nty, rest / a.nty, s);
 }
 
 // Shot of a batched run (device copy of the host descriptor).
@@ -615,7 +621,7 @@ constexpr int batch_waves()
     return (TY == 8 && (CM == CM_F16R || CM == CM_F32R)) ? 6 : 4;
 #endif
 }
-template <int TY, int CM>
+template <int TY, int CM, bool ISO>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(batch_waves<TY, CM>())))
 void tti_step_batch(const BatchArgs b)
 {
@@ -654,7 +660,7 @@ void tti_step_batch(const BatchArgs b)
     const int l = t - sh.item0;
     const int tix = l % sh.ntx;
     const int rest = l / sh.ntx;
-    step_tile<TY, CM, false>(a, ctl, tix, rest % sh.nty, rest / sh.nty, s);
+    step_tile<TY, CM, false, ISO>(a, ctl, tix, rest % sh.nty, rest / sh.nty, s);
 }
 
 }  // namespace fl

@@ -164,7 +164,9 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, Coef &
     } else if constexpr (CM == CM_BF16 || CM == CM_F16) {
         const uint32_t a0 = from_p(r.w[0]), a1 = from_p(r.w[1]);
         const uint32_t b0 = from_q(r.w[0]), b1 = from_q(r.w[1]);
-        const uint32_t own = r.w[2] >> (16 * fld);
+        // the own half by a select on the lane parity (a mask in SGPRs), not a per-lane
+        // shift count, which would hold a VGPR for the whole kernel
+        const uint32_t own = fld ? (r.w[2] >> 16) : (r.w[2] & 0xffffu);
         if constexpr (CM == CM_BF16) {
             k.c[0] = bf16_lo(a0); k.c[1] = bf16_hi(a0);
             k.c[2] = bf16_lo(a1); k.c[3] = bf16_hi(a1);
@@ -183,10 +185,10 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, Coef &
         // theta, q lanes v2pn and phi), v2pz and v2sz from the pair
         float own, vpz, vsz, t;
         if constexpr (CM == CM_F16R) {  // words (v2px, v2pn), (v2pz, v2sz), (theta, phi)
-            own = f16_to_f32((uint16_t)(r.w[0] >> (16 * fld)));
+            own = fld ? f16_to_f32((uint16_t)(r.w[0] >> 16)) : f16_to_f32((uint16_t)r.w[0]);
             vpz = f16_to_f32((uint16_t)r.w[1]);
             vsz = f16_to_f32((uint16_t)(r.w[1] >> 16));
-            t = (float)((r.w[2] >> (16 * fld)) & 0xffffu) * (1.0f / 65536.0f);
+            t = (float)(fld ? (r.w[2] >> 16) : (r.w[2] & 0xffffu)) * (1.0f / 65536.0f);
         } else {  // CM_F32R: lane words (own factor, v2pz | v2sz, theta | phi turns)
             own = __builtin_bit_cast(float, r.w[0]);
             vpz = __builtin_bit_cast(float, from_p(r.w[1]));
@@ -255,7 +257,9 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     const int zlo = a.bord + zc * a.zt;
     const int zhi = min(zlo + a.zt, a.sz - a.bord);
     const int nz = zhi - zlo, f0 = zlo - 4;
-    const int lane = threadIdx.x;
+    // lane id (= threadIdx.x, blocks are 64 wide) from mbcnt: a value the register
+    // allocator can recompute instead of keeping the work-item id alive or spilling it
+    const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // tile row (wave-uniform)
     // lane = 2 * column + field: a half wave reads 32 consecutive floats (16 cells x 2
     // fields) from global memory and LDS alike, free of LDS bank conflicts
@@ -319,15 +323,16 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     const int ow = wv * RW + lane;   // 9 x 9 window of the own (cell, field)
     const int oxw = wv * XW2 + lane; // 9-row column of the own (cell, field) in Dx
     constexpr int CW = 4 * RW + 8, CX = 4 * XW2;  // the own cell within them
-    int ohr[NH], ohx[NH];                     // y-halo rows of Dx done by this wave
+    int ohr[NH], ohx[NH];  // y-halo rows of Dx done by this wave, relative to ow / oxw
+                           // (uniform: no lane offsets kept besides ow, oxw)
     bool hrow[NH];
 #pragma unroll
     for (int m = 0; m < NH; m++) {
         const int j = wv + m * TY;  // halo row index 0..7 (rows 0-3 and TY+4..TY+7)
         hrow[m] = j < 8;
         const int r = j < 4 ? j : TY + j;
-        ohr[m] = r * RW + lane;
-        ohx[m] = r * XW2 + lane;
+        ohr[m] = (r - wv) * RW;
+        ohx[m] = (r - wv) * XW2;
     }
 
     // TWOPASS: first-derivative pair volumes of pass 1, over the chunk's input planes
@@ -417,8 +422,8 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
 #pragma unroll
             for (int m = 0; m < NH; m++)
                 if (hrow[m]) {
-                    if constexpr (TWOPASS) Xf[ohx[m]] = gxh[m];
-                    else Xf[ohx[m]] = der1_lds<!ISO>(Tf + ohr[m] + 8, 2, a.dxinv);
+                    if constexpr (TWOPASS) Xf[oxw + ohx[m]] = gxh[m];
+                    else Xf[oxw + ohx[m]] = der1_lds<!ISO>(Tf + ow + ohr[m] + 8, 2, a.dxinv);
                 }
         }
         // formed here, in this step: keeps the compiler from carrying the raw LDS values to
@@ -636,7 +641,7 @@ void tti_step_batch(const BatchArgs b)
         const int mid = (lo + hi + 1) >> 1;
         if (b.shots[mid].item0 <= t) lo = mid; else hi = mid - 1;
     }
-    const ShotDev &sh = b.shots[lo];
+    const ShotDev &sh = b.shots[__builtin_amdgcn_readfirstlane(lo)];
     StepArgs a;
     a.prev = sh.lv[b.parity];
     a.cur = sh.lv[1 - b.parity];

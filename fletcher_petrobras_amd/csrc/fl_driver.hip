@@ -90,6 +90,14 @@ static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_ev_used, g_ev_free;
 static hipEvent_t g_reg0 = nullptr, g_reg1 = nullptr;
 // tile width of the step kernel (columns per wave and field)
 static int tile_x() { return 32; }
+// First tile column at x = 0 (rows stay 128-byte aligned) or at x = bord when that needs
+// one tile column fewer over the update domain [bord, sx - bord).
+static int tile_xorg(int sx, int bord)
+{
+    const int t = tile_x();
+    return (sx - 2 * bord + t - 1) / t < (sx - bord + t - 1) / t ? bord : 0;
+}
+static int tiles_x(int sx, int bord) { return (sx - bord - tile_xorg(sx, bord) + tile_x() - 1) / tile_x(); }
 
 // Row pitch of the device layout: 16 cells = one 128-byte line of (p, q) pairs.
 static int pitch_of(int sx) { return (sx + 15) / 16 * 16; }
@@ -201,7 +209,7 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
 // ------------------------------------------------------------------ launch plumbing
 static int default_zt(int sx, int sy, int sz, int bord, int ty)
 {
-    const int ntx = (sx - bord + tile_x() - 1) / tile_x(), nty = (sy - 2 * bord + ty - 1) / ty;
+    const int ntx = tiles_x(sx, bord), nty = (sy - 2 * bord + ty - 1) / ty;
     const int nzu = sz - 2 * bord;
     const long long per_plane = (long long)ntx * nty;
     const long long target = 4096;  // measured best on 256 CUs: ~8 z chunks at 384^3
@@ -214,7 +222,8 @@ static int default_zt(int sx, int sy, int sz, int bord, int ty)
 static void fill_geom(StepArgs &a, int sx, int sy, int sz, int bord, int ty, int zt)
 {
     a.sx = sx; a.sy = sy; a.sz = sz; a.bord = bord;
-    a.ntx = (sx - bord + tile_x() - 1) / tile_x();  // tiles [0,TX), [TX,2TX) ... up to sx-bord
+    a.xorg = tile_xorg(sx, bord);  // tiles [xorg, xorg+TX), ... up to sx-bord
+    a.ntx = tiles_x(sx, bord);
     a.nty = (sy - 2 * bord + ty - 1) / ty;
     a.zt = zt;
     const int nzc = (sz - 2 * bord + zt - 1) / zt;

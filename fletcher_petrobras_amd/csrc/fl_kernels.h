@@ -99,6 +99,7 @@ struct StepArgs {
     float dxxinv, dyyinv, dzzinv, dxinv, dyinv, dzinv, dt;
     int sx, sy, sz, bord, px;
     int ntx, nty, zt, nblk;  // tiles along x, y; z chunk depth; logical blocks
+    int xorg;                // x of the first tile column (0 or bord)
     // step index it: it_host, or it_base + 1 + *it_dev inside a captured run, where
     // the kernel also publishes *it_next = *it_dev + 1 for the following step
     int it_host, it_base;

@@ -253,7 +253,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     constexpr int REC = rec_bytes(CM);
     constexpr int NH = (8 + TY - 1) / TY;      // y-halo rows of Dx per wave (at most)
     static_assert(NS % 64 == 0, "staged plane must be whole waves");
-    const int x0 = tix * TX, y0 = a.bord + tiy * TY;
+    const int x0 = a.xorg + tix * TX, y0 = a.bord + tiy * TY;
     const int zlo = a.bord + zc * a.zt;
     const int zhi = min(zlo + a.zt, a.sz - a.bord);
     const int nz = zhi - zlo, f0 = zlo - 4;
@@ -590,6 +590,7 @@ struct ShotDev {
     long long src_idx;  // pitched cell index
     int sx, sy, sz, px, rec_iz;
     int ntx, nty, item0, nitems;  // tiles along x, y; first work item; item count
+    int xorg;                     // x of the first tile column
 };
 
 struct BatchArgs {
@@ -615,15 +616,18 @@ __global__ void batch_insert(const ShotDev *shots, int nshots, int parity, float
 }
 
 // Batched shots: one workgroup per (shot, tile) item of the work list.
-// The batched kernel keeps a shot descriptor besides the tile state: 6 waves per SIMD
-// leave its FP32 and stored 16-bit forms spilling, the recomputed form does not.
+// The batched kernel keeps a shot descriptor besides the tile state (about 10 VGPRs more
+// than the single-grid kernel): held to 80 for 6 waves per SIMD, the 16-bit and recompute
+// forms spill a few values, nearly all outside the steady loop (BF16 on 32 shots: 83K ->
+// 94K MSamples/s against 4 waves); the FP32 form would reload a dozen per 10 planes and
+// keeps 4 waves.  FL_BATCH_WAVES overrides the 8-row value at build time.
 template <int TY, int CM>
 constexpr int batch_waves()
 {
-#ifdef FL_BATCH_W4
-    return 4;
+#ifdef FL_BATCH_WAVES
+    return TY == 8 ? FL_BATCH_WAVES : 4;
 #else
-    return (TY == 8 && (CM == CM_F16R || CM == CM_F32R)) ? 6 : 4;
+    return (TY == 8 && CM != CM_F32) ? 6 : 4;
 #endif
 }
 template <int TY, int CM, bool ISO>
@@ -650,7 +654,7 @@ void tti_step_batch(const BatchArgs b)
     a.dxxinv = b.dxxinv; a.dyyinv = b.dyyinv; a.dzzinv = b.dzzinv;
     a.dxinv = b.dxinv; a.dyinv = b.dyinv; a.dzinv = b.dzinv; a.dt = b.dt;
     a.sx = sh.sx; a.sy = sh.sy; a.sz = sh.sz; a.bord = b.bord; a.px = sh.px;
-    a.ntx = sh.ntx; a.nty = sh.nty; a.zt = b.zt; a.nblk = sh.nitems;
+    a.ntx = sh.ntx; a.nty = sh.nty; a.zt = b.zt; a.nblk = sh.nitems; a.xorg = sh.xorg;
     a.rec_iz = sh.rec_iz;
     a.src_idx = sh.src_idx;
     StepCtl ctl;

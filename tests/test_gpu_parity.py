@@ -127,6 +127,29 @@ def test_fp32_recompute_oracle_parity_heterogeneous_ragged(fl, oracle, drv):
     assert max(e) <= TOL
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp16r"])
+def test_tiles_from_x_bord(fl, oracle, drv, prec):
+    """sx = 72: the update domain [4, 68) takes two 32-cell tile columns from x = 4 against
+    three from x = 0, so the first tile column starts at x = 4.  FP32 against the oracle,
+    and both forms bitwise equal to a run whose tile rows differ."""
+    m, dt = hetero(fl, (48, 41, 30), 8)
+    shape = m[0].shape
+    assert shape[2] == 72
+    isrc = fl.index(shape, shape[2] // 2, shape[1] // 2, 4 + 8 + 3)
+    st = 60
+    outs = []
+    for tiles in ((8, 0), (16, 11)):
+        d = drv(variant="fused", precision=prec, tiles=tiles)
+        d.initialize(shape, D, dt, m)
+        d.run(st, isrc, graph_chunk=0)
+        outs.append(normal_part(d.field("pc")))
+        d.finalize()
+    assert np.array_equal(outs[0], outs[1])
+    if prec == "fp32":
+        r = oracle.run("der1der1", shape, 4, D, dt, st, isrc, oracle.precompute(m))
+        assert rel_l2(outs[0], r["pc"]) <= TOL
+
+
 def test_fp64_oracle_error_reported(fl, oracle, drv):
     m, dt = hetero(fl, (48, 40, 36), 6)
     shape = m[0].shape

@@ -261,6 +261,7 @@ static int launch_steps(const StepArgs &a, int ty, int cm, bool tp, hipStream_t 
     }
     FL_ZM(4)
     FL_ZM(8)
+    FL_ZM(12)
     FL_ZM(16)
 #undef FL_ZM
 #undef FL_ZM1
@@ -589,7 +590,8 @@ extern "C" int fl_set_sync(int on)
 extern "C" int fl_set_tiles(int ty, int zt)
 {
     if (ty == 0) ty = 8;
-    if (ty != 4 && ty != 8 && ty != 16) return fail(FL_ERR_ARG, "tile rows must be 4, 8 or 16 (got %d)", ty);
+    if (ty != 4 && ty != 8 && ty != 12 && ty != 16)
+        return fail(FL_ERR_ARG, "tile rows must be 4, 8, 12 or 16 (got %d)", ty);
     if (zt < 0) return fail(FL_ERR_ARG, "zt must be >= 0");
     g_tiles.ty = ty;
     g_tiles.zt = zt;
@@ -884,7 +886,7 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
     const Tiles keep = g_tiles;
     double best = 1e30;
     Tiles bt = keep;
-    for (int cty : {4, 8, 16}) {
+    for (int cty : {4, 8, 12, 16}) {
         for (int czt : zt_candidates(std::min(D.sz - 2 * D.bord, max_zt(D.px, D.sy)))) {
             g_tiles.ty = cty;
             g_tiles.zt = czt;

@@ -552,8 +552,8 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
 }
 
 // Waves per SIMD the register allocator is asked for.  With 8-row tiles the LDS rings
-// (51 KB per workgroup) allow three workgroups per CU: 6 waves per SIMD, <= 80 VGPRs,
-// which the 16-bit and recompute forms meet.  The FP32 records keep two workgroups
+// (51 KB per workgroup) allow three workgroups per CU, with 12-row tiles (63 KB) two:
+// 6 waves per SIMD, <= 80 VGPRs, which the 16-bit and recompute forms meet.  The FP32 records keep two workgroups
 // (4 waves, 85 VGPRs): held to 80 they spill one value per step and run 7% slower at
 // 344^3.  4- and 16-row tiles are held to 4 waves per SIMD by their LDS.
 // FL_STEP_WAVES overrides the 8-row value at build time.
@@ -563,7 +563,7 @@ constexpr int step_waves()
 #ifdef FL_STEP_WAVES
     return TY == 8 ? FL_STEP_WAVES : 4;
 #else
-    return (TY == 8 && CM != CM_F32) ? 6 : 4;
+    return ((TY == 8 || TY == 12) && CM != CM_F32) ? 6 : 4;
 #endif
 }
 
@@ -627,7 +627,7 @@ constexpr int batch_waves()
 #ifdef FL_BATCH_WAVES
     return TY == 8 ? FL_BATCH_WAVES : 4;
 #else
-    return (TY == 8 && CM != CM_F32) ? 6 : 4;
+    return ((TY == 8 || TY == 12) && CM != CM_F32) ? 6 : 4;
 #endif
 }
 template <int TY, int CM, bool ISO>

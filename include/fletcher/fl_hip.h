@@ -57,7 +57,7 @@ int fl_set_precision(int precision); /* used by the next DRIVER_Initialize */
 int fl_get_precision(void);
 int fl_set_sync(int on);           /* 1 (default): DRIVER_Propagate waits for the step */
 
-/* Tile shape of the z-marching kernel: ty rows (4, 8 or 16) of 32 cells, z chunks of
+/* Tile shape of the z-marching kernel: ty rows (4, 8, 12 or 16) of 32 cells, z chunks of
  * zt planes.  0 selects the built-in default for the grid. */
 int fl_set_tiles(int ty, int zt);
 int fl_get_tiles(int *ty, int *zt);

@@ -96,7 +96,9 @@ def test_batch_continues_across_runs_and_tiles(fl):
     try:
         b.run(30, graph_chunk=6)
         b.set_tiles(4, 17)
-        b.run(21, it0=30, graph_chunk=0)
+        b.run(11, it0=30, graph_chunk=0)
+        b.set_tiles(12, 9)
+        b.run(10, it0=41, graph_chunk=5)
         got = [b.field(k, "pc") for k in range(2)]
     finally:
         b.destroy()

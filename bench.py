@@ -81,7 +81,7 @@ def cpu_baseline(model, shape, dt, isrc, steps, threads):
 
 def tiles_for(d, spec):
     if spec == "auto":
-        return d.autotune(3)
+        return d.autotune(10)
     ty, zt = (int(v) for v in spec.split(","))
     d.set_tiles(ty, zt)
     return ty, zt
@@ -203,7 +203,8 @@ def batched(F, np, precisions, steps, nshots):
         t0 = time.time()
         b = F.Batch(sh, D, dt, precision=prec)
         try:
-            b.autotune(4)
+            b.autotune(6)
+            tiles = b.tiles()
             b.run(steps, graph_chunk=50, rec_every=4, max_traces=(steps + 3) // 4, src=src)
             ms = b.last_ms()
             tr = b.traces(0)
@@ -211,7 +212,8 @@ def batched(F, np, precisions, steps, nshots):
         finally:
             b.destroy()
         value = 1e-6 * samples * steps / (ms * 1e-3)
-        out[prec] = {"value": round(value, 1), "ms_per_step": round(ms / steps, 4), "finite": finite}
+        out[prec] = {"value": round(value, 1), "ms_per_step": round(ms / steps, 4), "tiles": list(tiles),
+                     "finite": finite}
         log("batched %s: %.0f MS/s (%.1f ms for %d steps; %.1f s incl. upload and tuning)"
             % (prec, value, ms, steps, time.time() - t0))
     return out

@@ -85,6 +85,7 @@ def lib():
         "fl_bytes_per_cell": (i, []),
         "fl_batch_create": (i, [i, vp, i, f, f, f, f, i]),
         "fl_batch_set_tiles": (i, [i, i]),
+        "fl_batch_get_tiles": (i, [C.c_void_p, C.c_void_p]),
         "fl_batch_run": (i, [i, i, _f32p, i, i, i]),
         "fl_batch_get_field": (i, [i, i, _f32p]),
         "fl_batch_get_traces": (i, [i, _f32p, i]),
@@ -456,6 +457,11 @@ class Batch:
 
     def set_tiles(self, ty, zt):
         _check(self.L.fl_batch_set_tiles(ty, zt))
+
+    def tiles(self):
+        ty, zt = C.c_int(), C.c_int()
+        _check(self.L.fl_batch_get_tiles(C.byref(ty), C.byref(zt)))
+        return ty.value, zt.value
 
     def last_ms(self):
         ms = C.c_double()

@@ -886,6 +886,9 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
     const Tiles keep = g_tiles;
     double best = 1e30;
     Tiles bt = keep;
+    // two passes over the candidates, the fastest single measurement wins: one pass
+    // leaves the early candidates timed while the clocks still ramp
+    for (int pass = 0; pass < 2 && rc == FL_OK; pass++)
     for (int cty : {4, 8, 12, 16}) {
         for (int czt : zt_candidates(std::min(D.sz - 2 * D.bord, max_zt(D.px, D.sy)))) {
             g_tiles.ty = cty;

@@ -561,7 +561,7 @@ template <int TY, int CM>
 constexpr int step_waves()
 {
 #ifdef FL_STEP_WAVES
-    return TY == 8 ? FL_STEP_WAVES : 4;
+    return (TY == 8 || TY == 12) ? FL_STEP_WAVES : 4;
 #else
     return ((TY == 8 || TY == 12) && CM != CM_F32) ? 6 : 4;
 #endif

@@ -113,6 +113,7 @@ typedef struct {
 int fl_batch_create(int nshots, const fl_shot_t *shots, int bord, float dx, float dy, float dz,
                     float dt, int precision);
 int fl_batch_set_tiles(int ty, int zt);
+int fl_batch_get_tiles(int *ty, int *zt);  /* zt as used (0 resolved to the default) */
 /* Steps it0+1..it0+nsteps for every shot; src as in fl_run_steps; traces every
  * rec_every steps (0: none) into max_traces slots per shot. */
 int fl_batch_run(int it0, int nsteps, const float *src, int graph_chunk, int rec_every,

@@ -222,8 +222,8 @@ def batched(F, np, precisions, steps, nshots):
 def large(F, np, args):
     """The 1024x1024x768 model near HBM capacity: seeded layered model with 12 dipping layers
     (vpz 1500-5000 m/s, eps in [0,0.3], delta in [0,0.2] <= eps, smooth theta and phi), absorb
-    32, dx=dy=dz=10 m (array 1096x1096x840), Courant 0.4.  FP32 drop-in and FP16 recompute,
-    each against the device FP64 reference over the whole grid."""
+    32, dx=dy=dz=10 m (array 1096x1096x840), Courant 0.4.  FP32 drop-in, FP32 recompute and
+    FP16 recompute, each against the device FP64 reference over the whole grid."""
     d3 = (10.0, 10.0, 10.0)
     t0 = time.time()
     model = F.model_layered((1024, 1024, 768), 32, seed=1024)
@@ -239,7 +239,7 @@ def large(F, np, args):
         variant="fused", precision="fp32", tiles=args.tiles, warmup=0, steps=steps), d3)
     err = rel_l2(np, main_r.pop("pc"), ref["pc_steps"])
     graphs = {}
-    for prec in ("fp32", "fp16r"):
+    for prec in ("fp32", "fp32r", "fp16r"):
         graphs[prec] = graph_run(F, np, model, shape, dt, isrc, "fused", prec, steps, rec_iz, args.tiles, ref, d3)
         graphs[prec]["vs_fp32_dropin"] = round(graphs[prec]["value"] / main_r["value"], 3)
     del ref

@@ -19,7 +19,8 @@
 // in it, and nothing read in it is overwritten before the next barrier.
 //
 // Memory.  The levels hold (p, q) pairs per cell (pitched rows), so a wave's 32 cells x 2
-// fields are 64 consecutive floats; the coefficients are one record per cell.  Every
+// fields are 64 consecutive floats; the coefficients are one record per cell, whose two
+// halves the p and q lanes of the cell load and exchange (DPP broadcast).  Every
 // stream goes through a buffer resource over the chunk's span with a
 // wave-uniform plane offset (soffset): one SGPR add per plane, no 64-bit address math.
 // Lanes outside the grid carry an offset past the range, so their loads return 0 and
@@ -344,7 +345,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     float qc_[10], qx_[10], qy_[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) qc_[k] = qx_[k] = qy_[k] = 0.0f;
-    // output-plane streams loaded one step ahead (set = parity of the consuming step)
+    // output-plane streams loaded prefetch_steps() ahead (set = parity of the consuming step)
     CoefRec<CM> kp[2];
     float ppv[2] = {0.0f, 0.0f};
 
@@ -556,7 +557,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
 // 6 waves per SIMD, <= 80 VGPRs, which the 16-bit and recompute forms meet.  The FP32 records keep two workgroups
 // (4 waves, 85 VGPRs): held to 80 they spill one value per step and run 7% slower at
 // 344^3.  4- and 16-row tiles are held to 4 waves per SIMD by their LDS.
-// FL_STEP_WAVES overrides the 8-row value at build time.
+// FL_STEP_WAVES overrides the 8- and 12-row value at build time.
 template <int TY, int CM>
 constexpr int step_waves()
 {
@@ -620,12 +621,12 @@ __global__ void batch_insert(const ShotDev *shots, int nshots, int parity, float
 // than the single-grid kernel): held to 80 for 6 waves per SIMD, the 16-bit and recompute
 // forms spill a few values, nearly all outside the steady loop (BF16 on 32 shots: 83K ->
 // 94K MSamples/s against 4 waves); the FP32 form would reload a dozen per 10 planes and
-// keeps 4 waves.  FL_BATCH_WAVES overrides the 8-row value at build time.
+// keeps 4 waves.  FL_BATCH_WAVES overrides the 8- and 12-row value at build time.
 template <int TY, int CM>
 constexpr int batch_waves()
 {
 #ifdef FL_BATCH_WAVES
-    return TY == 8 ? FL_BATCH_WAVES : 4;
+    return (TY == 8 || TY == 12) ? FL_BATCH_WAVES : 4;
 #else
     return ((TY == 8 || TY == 12) && CM != CM_F32) ? 6 : 4;
 #endif

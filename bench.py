@@ -22,14 +22,16 @@ sources) timed on a few steps of the same grid on the host cores.
 extras (skipped with --quick), each timed over its whole run with inputs resident:
   twopass   the der1der1 two-pass variant on the same grid (104 algorithmic B/sample)
   fp32_graph / mixed   FP32, the FP32 in-kernel recompute (fp32r: FP32 arithmetic, 24-byte
-            records), BF16 / FP16 coefficients and the FP16 in-kernel recompute on the
-            same grid, receivers on plane z=28 every 4 steps, 50-step graph chunks
+            records), BF16 / FP16 coefficients, the FP16 in-kernel recompute and its
+            16-bit fixed-point variant (q16r) on the same grid, receivers on plane z=28
+            every 4 steps, 50-step graph chunks
   rel_l2_vs_fp64   relative L2 (in FP64) of the receiver traces and the final pc against
             the device FP64 reference (fl_ref64_*: the der1der1 loop in double, pinned
             bitwise to the FP64 oracle by the tests) on the same grid and steps
   batched   32 shots on uneven (128..256)^2 x 320 apertures cut from one seeded
             512x512x320 model (absorb 16), 800 steps in 50-step graphs, autotuned
-            tiles, FP32, BF16 and FP16-recompute coefficients; receivers every 4 steps
+            tiles, FP32, BF16, FP16-recompute and fixed-point-recompute coefficients;
+            receivers every 4 steps
 """
 import argparse
 import json
@@ -222,8 +224,8 @@ def batched(F, np, precisions, steps, nshots):
 def large(F, np, args):
     """The 1024x1024x768 model near HBM capacity: seeded layered model with 12 dipping layers
     (vpz 1500-5000 m/s, eps in [0,0.3], delta in [0,0.2] <= eps, smooth theta and phi), absorb
-    32, dx=dy=dz=10 m (array 1096x1096x840), Courant 0.4.  FP32 drop-in, FP32 recompute and
-    FP16 recompute, each against the device FP64 reference over the whole grid."""
+    32, dx=dy=dz=10 m (array 1096x1096x840), Courant 0.4.  FP32 drop-in, FP32, FP16 and
+    fixed-point recompute, each against the device FP64 reference over the whole grid."""
     d3 = (10.0, 10.0, 10.0)
     t0 = time.time()
     model = F.model_layered((1024, 1024, 768), 32, seed=1024)
@@ -239,7 +241,7 @@ def large(F, np, args):
         variant="fused", precision="fp32", tiles=args.tiles, warmup=0, steps=steps), d3)
     err = rel_l2(np, main_r.pop("pc"), ref["pc_steps"])
     graphs = {}
-    for prec in ("fp32", "fp32r", "fp16r"):
+    for prec in ("fp32", "fp32r", "fp16r", "q16r"):
         graphs[prec] = graph_run(F, np, model, shape, dt, isrc, "fused", prec, steps, rec_iz, args.tiles, ref, d3)
         graphs[prec]["vs_fp32_dropin"] = round(graphs[prec]["value"] / main_r["value"], 3)
     del ref
@@ -270,7 +272,7 @@ def main():
     ap.add_argument("--nx", type=int, default=344)
     ap.add_argument("--absorb", type=int, default=16)
     ap.add_argument("--variant", default="fused", choices=["fused", "direct", "twopass"])
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp16r", "fp32r"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r"])
     ap.add_argument("--tiles", default="auto", help="ty,zt or 'auto' (autotuner)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -319,13 +321,13 @@ def main():
                   "fp32_graph": graph_run(F, np, model, shape, dt, isrc, "fused", "fp32", args.steps,
                                           rec_iz, "auto", ref),
                   "mixed": {}}
-        for prec in ("fp32r", "bf16", "fp16", "fp16r"):
+        for prec in ("fp32r", "bf16", "fp16", "fp16r", "q16r"):
             r = graph_run(F, np, model, shape, dt, isrc, "fused", prec, args.steps, rec_iz, "auto", ref)
             r["vs_fp32_dropin"] = round(r["value"] / value, 3)
             extras["mixed"][prec] = r
         del ref
-        bt = batched(F, np, ("fp32", "bf16", "fp16r"), args.batch_steps, args.shots)
-        for prec in ("fp32", "bf16", "fp16r"):
+        bt = batched(F, np, ("fp32", "bf16", "fp16r", "q16r"), args.batch_steps, args.shots)
+        for prec in ("fp32", "bf16", "fp16r", "q16r"):
             bt[prec]["vs_fp32_dropin"] = round(bt[prec]["value"] / value, 3)
         extras["batched"] = bt
 

@@ -70,6 +70,7 @@ struct Device {
     int lc = 1;             // level holding pc, qc; lv[1 - lc] holds pp, qp
     void *coef = nullptr;   // coefficient records, pitched like the levels
     int cm = CM_F32;
+    float vq = 0, vqs = 0;  // CM_Q16R: fixed-point unit (v2px, v2pz, v2pn); v2sz unit / vq
     float *dvx = nullptr, *dvy = nullptr;  // two-pass (pDx, qDx), (pDy, qDy) pairs
     float *trace = nullptr;
     int rec_iz = -1, rec_every = 0, rec_slots = 0;
@@ -143,7 +144,7 @@ static int dev_alloc(void **p, size_t bytes, long long *acct)
     return FL_OK;
 }
 
-static int coef_count(int cm) { return (cm == CM_F16R || cm == CM_F32R) ? 6 : 10; }
+static int coef_count(int cm) { return (cm == CM_F16R || cm == CM_F32R || cm == CM_Q16R) ? 6 : 10; }
 static int coef_elem(int cm) { return (cm == CM_F32 || cm == CM_F32R) ? 4 : 2; }
 
 extern "C" int fl_bytes_per_cell(void)
@@ -158,9 +159,10 @@ extern "C" int fl_bytes_per_cell(void)
 // values (FP32 or the 16-bit forms, or the recompute inputs) in the slot order
 // rec_order32 / rec_order16 / rec_order_r32 / rec_order_r16 (record of
 // coef_count * coef_elem bytes); z slabs of at most 64 MiB per array on the host.
+// vmax: the CM_Q16R full scales (fl_recompute_q16_range over the whole grid).
 static int upload_coefs(const float *vpz, const float *vsv, const float *eps, const float *del,
                         const float *phi, const float *theta, float dt, int cm, void *dst, int sx,
-                        int sy, int sz, int px)
+                        int sy, int sz, int px, const double *vmax = nullptr)
 {
     const long long sxy = (long long)sx * sy;
     const int slab = std::max(1, (int)std::min<long long>(sz, (64ll << 20) / (sxy * 4)));
@@ -193,11 +195,15 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
             for (int k = 0; k < nk; k++) c[k] = h16.data() + k * m;
             if (cm == CM_F16R)
                 fl_pack_recompute16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt, c);
+            else if (cm == CM_Q16R)
+                fl_pack_recompute_q16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
+                                      vmax, c);
             else
                 fl_pack_coefs16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
                                 cm == CM_BF16, c);
             const uint16_t *co[10];
-            for (int k = 0; k < nk; k++) co[k] = c[cm == CM_F16R ? rec_order_r16(k) : rec_order16(k)];
+            for (int k = 0; k < nk; k++)
+                co[k] = c[(cm == CM_F16R || cm == CM_Q16R) ? rec_order_r16(k) : rec_order16(k)];
             fl_interleave16(m, nk, co, r16.data());
             HIPCHK(hipMemcpy2D((char *)dst + (size_t)z0 * sy * px * rec, px * rec, r16.data(), sx * rec,
                                sx * rec, (size_t)sy * (z1 - z0), hipMemcpyHostToDevice));
@@ -257,6 +263,7 @@ static int launch_steps(const StepArgs &a, int ty, int cm, bool tp, hipStream_t 
             FL_ZM1(TY, CM_F16)                                             \
             FL_ZM1(TY, CM_F16R)                                            \
             FL_ZM1(TY, CM_F32R)                                            \
+            FL_ZM1(TY, CM_Q16R)                                            \
         }                                                                  \
     }
     FL_ZM(4)
@@ -285,6 +292,8 @@ static StepArgs base_args()
     a.dyinv = 1.0f / D.dy;
     a.dzinv = 1.0f / D.dz;
     a.dt = D.dt;
+    a.vq = D.vq;
+    a.vqs = D.vqs;
     const int zt = g_tiles.zt > 0 ? std::min(std::min(g_tiles.zt, D.sz - 2 * D.bord), max_zt(D.px, D.sy))
                                   : default_zt(D.sx, D.sy, D.sz, D.bord, g_tiles.ty);
     fill_geom(a, D.sx, D.sy, D.sz, D.bord, g_tiles.ty, zt);
@@ -452,7 +461,13 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
     }
     const size_t cbytes = (size_t)coef_count(D.cm) * D.np * coef_elem(D.cm);
     if ((rc = dev_alloc(&D.coef, cbytes, &D.bytes))) { free_device(); return rc; }
-    if ((rc = upload_coefs(vpz, vsv, eps, del, phi, theta, dt, D.cm, D.coef, sx, sy, sz, D.px))) {
+    double vmax[2] = {0.0, 0.0};
+    if (D.cm == CM_Q16R) {
+        fl_recompute_q16_range(D.n, vpz, vsv, eps, del, dt, vmax);
+        D.vq = (float)(vmax[0] / 65535.0);
+        D.vqs = vmax[0] > 0.0 ? (float)(vmax[1] / vmax[0]) : 0.0f;
+    }
+    if ((rc = upload_coefs(vpz, vsv, eps, del, phi, theta, dt, D.cm, D.coef, sx, sy, sz, D.px, vmax))) {
         free_device();
         return rc;
     }
@@ -463,7 +478,7 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
 
 // Selection through the environment for front-ends that only call driver.h (the
 // reference's main.c/model.c): FL_VARIANT = fused|direct|twopass, FL_PRECISION =
-// fp32|bf16|fp16|fp16r|fp32r, applied at DRIVER_Initialize when set; FL_TUNE = cache file, the
+// fp32|bf16|fp16|fp16r|fp32r|q16r, applied at DRIVER_Initialize when set; FL_TUNE = cache file, the
 // autotuner runs (or reads the cache) right after it.
 static int env_pick(const char *name, const char *const *opts, int n)
 {
@@ -483,8 +498,8 @@ extern "C" void DRIVER_Initialize(const int sx, const int sy, const int sz, cons
                                   float *__restrict__ qc)
 {
     static const char *const variants[] = {"fused", "direct", "twopass"};
-    static const char *const precs[] = {"fp32", "bf16", "fp16", "fp16r", "fp32r"};
-    const int v = env_pick("FL_VARIANT", variants, 3), p = env_pick("FL_PRECISION", precs, 5);
+    static const char *const precs[] = {"fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r"};
+    const int v = env_pick("FL_VARIANT", variants, 3), p = env_pick("FL_PRECISION", precs, 6);
     if (v == -2 || p == -2) {
         fail(FL_ERR_ARG, "FL_VARIANT / FL_PRECISION hold an unknown value");
         return;
@@ -575,7 +590,7 @@ extern "C" int fl_get_variant(void) { return g_variant; }
 
 extern "C" int fl_set_precision(int p)
 {
-    if (p < FL_PREC_FP32 || p > FL_PREC_FP32_RECOMPUTE) return fail(FL_ERR_ARG, "unknown precision %d", p);
+    if (p < FL_PREC_FP32 || p > FL_PREC_Q16_RECOMPUTE) return fail(FL_ERR_ARG, "unknown precision %d", p);
     g_precision = p;
     return FL_OK;
 }

@@ -314,6 +314,47 @@ void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const fl
     }
 }
 
+/* Full scales of the fixed-point velocity factors over n cells (FP64, times dt^2):
+ * vmax[0] the largest of v2px, v2pz, v2pn, vmax[1] the largest v2sz. */
+void fl_recompute_q16_range(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                            const float *delta, double dt, double *vmax)
+{
+    double m0 = 0.0, m1 = 0.0;
+    const double dt2 = dt * dt;
+#pragma omp parallel for schedule(static) reduction(max : m0, m1)
+    for (int64_t i = 0; i < n; i++) {
+        const double v2pz = (double)vpz[i] * vpz[i];
+        const double a = v2pz * (1.0 + 2.0 * epsilon[i]) * dt2, b = v2pz * dt2,
+                     c = v2pz * (1.0 + 2.0 * delta[i]) * dt2, s = (double)vsv[i] * vsv[i] * dt2;
+        if (a > m0) m0 = a;
+        if (b > m0) m0 = b;
+        if (c > m0) m0 = c;
+        if (s > m1) m1 = s;
+    }
+    vmax[0] = m0;
+    vmax[1] = m1;
+}
+
+void fl_pack_recompute_q16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                           const float *delta, const float *phi, const float *theta, double dt,
+                           const double *vmax, uint16_t *const *out)
+{
+    const double pi = 3.14159265358979323846, dt2 = dt * dt;
+    const double q0 = vmax[0] > 0.0 ? 65535.0 / vmax[0] : 0.0, q1 = vmax[1] > 0.0 ? 65535.0 / vmax[1] : 0.0;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const double v2pz = (double)vpz[i] * vpz[i];
+        const double f[4] = {v2pz * (1.0 + 2.0 * epsilon[i]), v2pz, (double)vsv[i] * vsv[i],
+                             v2pz * (1.0 + 2.0 * delta[i])};
+        for (int k = 0; k < 4; k++) {
+            const double u = nearbyint(f[k] * dt2 * (k == 2 ? q1 : q0));
+            out[k][i] = (uint16_t)(u < 0.0 ? 0.0 : (u > 65535.0 ? 65535.0 : u));
+        }
+        out[4][i] = fl_angle_u16((double)theta[i] / (2.0 * pi));
+        out[5][i] = fl_angle_u16((double)phi[i] / (2.0 * pi));
+    }
+}
+
 void fl_pack_recompute32(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, float *const *out)
 {

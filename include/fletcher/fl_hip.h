@@ -32,7 +32,9 @@ enum { FL_VARIANT_FUSED = 0, FL_VARIANT_DIRECT = 1, FL_VARIANT_TWOPASS = 2 };
  * store the same 10 volumes (v2* pre-multiplied by dt^2, evaluated in FP64 and
  * rounded once); FP16_RECOMPUTE stores the 4 velocity factors (times dt^2) in FP16
  * and theta, phi as 16-bit fixed point turns, and rebuilds the 6 angular coefficients
- * per cell; FP32_RECOMPUTE stores the 4 precomp.h velocity factors bit for bit and
+ * per cell; Q16_RECOMPUTE stores the same inputs with the velocity factors in 16-bit fixed
+ * point (unit: the grid's largest factor / 65535), 16 times finer than FP16 at the top of
+ * the range; FP32_RECOMPUTE stores the 4 precomp.h velocity factors bit for bit and
  * theta, phi as FP32 turns, and rebuilds the angular coefficients in FP32 (24 bytes per
  * cell instead of 40).  Wavefields stay FP32.  The recompute forms and the 16-bit forms
  * run with the fused variant. */
@@ -41,7 +43,8 @@ enum {
     FL_PREC_BF16 = 1,
     FL_PREC_FP16 = 2,
     FL_PREC_FP16_RECOMPUTE = 3,
-    FL_PREC_FP32_RECOMPUTE = 4
+    FL_PREC_FP32_RECOMPUTE = 4,
+    FL_PREC_Q16_RECOMPUTE = 5
 };
 
 const char *fl_last_error(void);
@@ -49,7 +52,7 @@ int fl_status(void);      /* code of the last failed call, FL_OK after fl_clear_
 void fl_clear_error(void);
 
 /* Front-ends that call only driver.h select with the environment instead, read by
- * DRIVER_Initialize: FL_VARIANT=fused|direct|twopass, FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r,
+ * DRIVER_Initialize: FL_VARIANT=fused|direct|twopass, FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r|q16r,
  * FL_TUNE=<cache file> (autotune right after initialisation). */
 int fl_set_variant(int variant);   /* takes effect at the next step */
 int fl_get_variant(void);

@@ -1,6 +1,6 @@
 """Mixed-precision coefficient storage and the batched multi-shot engine on the GPU.
 
-* BF16 / FP16 / FP16-recompute / FP32-recompute coefficients against the FP64 oracle
+* BF16 / FP16 / FP16-recompute / fixed-point-recompute / FP32-recompute coefficients against the FP64 oracle
   (errors printed and bounded; FP32 stays the 1e-5 drop-in);
 * every shot of a batch equals the same shot run alone through the drop-in driver
   (same per-cell arithmetic: bitwise on normal floats), for FP32 and BF16 coefficients,
@@ -39,7 +39,7 @@ def mixed_case(fl, oracle):
 
 
 @pytest.mark.parametrize("prec,bound", [("fp32", 5e-5), ("fp32r", 5e-5), ("fp16", 2e-3), ("fp16r", 3e-3),
-                                        ("bf16", 3e-2)])
+                                        ("q16r", 6e-4), ("bf16", 3e-2)])
 def test_precision_error_vs_fp64_oracle(fl, mixed_case, prec, bound):
     c = mixed_case
     d = fl.Driver(variant="fused", precision=prec)
@@ -66,7 +66,8 @@ def _batch_setup(fl, nshots=3, absorb=6):
     return sh, dt
 
 
-@pytest.mark.parametrize("prec,chunk", [("fp32", 0), ("fp32", 10), ("bf16", 10), ("fp16r", 0), ("fp32r", 10)])
+@pytest.mark.parametrize("prec,chunk", [("fp32", 0), ("fp32", 10), ("bf16", 10), ("fp16r", 0), ("fp32r", 10),
+                                        ("q16r", 10)])
 def test_batch_equals_single_shots(fl, prec, chunk):
     shots, dt = _batch_setup(fl)
     st, every = 48, 4

@@ -244,6 +244,33 @@ def test_pack_recompute32_inputs(fl):
         assert np.abs(rebuilt[j] - c64[j]).max() < 1e-6, j
 
 
+def test_pack_recompute_q16_inputs(fl):
+    """Fixed-point recompute inputs: the FP64 velocity factors times dt^2 in units of
+    vmax / 65535, rounded to nearest, with vmax the largest of v2px, v2pz, v2pn over the grid
+    for those three and the largest v2sz for v2sz; the angles as u16 turns (the
+    FP16-recompute encoding)."""
+    m = fl.model_gradient((16, 14, 12), 4, seed=11)
+    n = m[0].size
+    dt = 8.0e-4
+    c64 = fl.precompute(m, "f64")
+    dt2 = dt * dt
+    want = [c64[6] * dt2, c64[7] * dt2, c64[8] * dt2, c64[9] * dt2]
+    vmax = (C.c_double * 2)()
+    fl.lib().fl_recompute_q16_range(n, m[0], m[1], m[2], m[3], dt, vmax)
+    assert vmax[0] == max(float(want[k].max()) for k in (0, 1, 3))
+    assert vmax[1] == float(want[2].max()) > 0
+    out = [np.empty(m[0].shape, np.uint16) for _ in range(6)]
+    fl.lib().fl_pack_recompute_q16(n, *m, dt, vmax, (C.c_void_p * 6)(*[o.ctypes.data for o in out]))
+    for k in range(4):
+        unit = (vmax[1] if k == 2 else vmax[0]) / 65535.0
+        assert np.array_equal(out[k], np.rint(want[k] / unit).astype(np.uint16)), k
+        assert np.abs(out[k] * unit - want[k]).max() <= 0.5 * unit * (1 + 1e-9)
+    assert out[0].max() == 65535 and out[2].max() == 65535
+    ref16 = [np.empty(m[0].shape, np.uint16) for _ in range(6)]
+    fl.lib().fl_pack_recompute16(n, *m, dt, (C.c_void_p * 6)(*[o.ctypes.data for o in ref16]))
+    assert np.array_equal(out[4], ref16[4]) and np.array_equal(out[5], ref16[5])
+
+
 def test_pack_recompute16_inputs(fl):
     """FP16 recompute inputs: the FP64 velocity factors times dt^2, rounded once (numpy's
     correctly rounded float64 -> float16), and the angles as u16 turns."""

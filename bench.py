@@ -274,7 +274,7 @@ def main():
     ap.add_argument("--variant", default="fused", choices=["fused", "direct", "twopass"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r"])
     ap.add_argument("--tiles", default="auto", help="ty,zt or 'auto' (autotuner)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline line only, no extras")
     ap.add_argument("--shots", type=int, default=32)

@@ -6,7 +6,8 @@
  *
  * Semantics of this HIP back-end (the reference CUDA back-end's, cuda_driver.c):
  *  - DRIVER_Initialize precomputes the 10 coefficient volumes from the model
- *    (precomp.h:14-69) and uploads them with the fields to the GPU.  It reads
+ *    (precomp.h:14-69), or their 16-bit or recompute forms under another precision
+ *    (fl_hip.h), and uploads them with the fields to the GPU.  It reads
  *    pp/pc/qp/qc once (the reference front-end passes zeros).
  *  - DRIVER_InsertSource adds src to the device pc and qc at iSource
  *    (OpenMP/openmp_insertsource.c:7-12, CUDA/cuda_insertsource.cu:28-32).

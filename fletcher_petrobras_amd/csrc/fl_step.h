@@ -143,6 +143,14 @@ __device__ __forceinline__ uint32_t from_q(uint32_t v)
 {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xF5, 0xF, 0xF, true);
 }
+__device__ __forceinline__ float from_pf(float v)
+{
+    return __builtin_bit_cast(float, from_p(__builtin_bit_cast(uint32_t, v)));
+}
+__device__ __forceinline__ float from_qf(float v)
+{
+    return __builtin_bit_cast(float, from_q(__builtin_bit_cast(uint32_t, v)));
+}
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
 
@@ -202,22 +210,19 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, float 
             vsz = __builtin_bit_cast(float, from_q(r.w[1]));
             t = __builtin_bit_cast(float, r.w[2]);
         }
-        // the angular coefficients of precomp.h:25-38; v_sin / v_cos take turns, each
-        // lane evaluates its own angle and the pair exchanges them
+        // the angular coefficients of precomp.h:25-38; v_sin / v_cos take turns.  Each
+        // lane evaluates the sine and cosine of its own angle (theta on p lanes, phi on q
+        // lane, and the products of one angle too (sin^2, cos^2, sin 2a); each coefficient
+        // is then one product of a p-lane and a q-lane value
         const float sn = __builtin_amdgcn_sinf(t), cs = __builtin_amdgcn_cosf(t);
-        const float st = __builtin_bit_cast(float, from_p(__builtin_bit_cast(uint32_t, sn)));
-        const float ct = __builtin_bit_cast(float, from_p(__builtin_bit_cast(uint32_t, cs)));
-        const float sp = __builtin_bit_cast(float, from_q(__builtin_bit_cast(uint32_t, sn)));
-        const float cp = __builtin_bit_cast(float, from_q(__builtin_bit_cast(uint32_t, cs)));
-        const float st2 = st * st;
-        const float s2t = 2.0f * st * ct;
-        const float s2p = 2.0f * sp * cp;
-        k.c[0] = st2 * cp * cp;
-        k.c[1] = st2 * sp * sp;
-        k.c[2] = ct * ct;
-        k.c[3] = st2 * s2p;
-        k.c[4] = s2t * sp;
-        k.c[5] = s2t * cp;
+        const float s2 = sn * sn, c2 = cs * cs, sc2 = (sn + sn) * cs;
+        const float st2 = from_pf(s2), s2t = from_pf(sc2);  // sin^2 theta, sin 2theta
+        k.c[0] = st2 * from_qf(c2);   // sin^2 theta cos^2 phi
+        k.c[1] = st2 * from_qf(s2);   // sin^2 theta sin^2 phi
+        k.c[2] = from_pf(c2);         // cos^2 theta
+        k.c[3] = st2 * from_qf(sc2);  // sin^2 theta sin 2phi
+        k.c[4] = s2t * from_qf(sn);   // sin 2theta sin phi
+        k.c[5] = s2t * from_qf(cs);   // sin 2theta cos phi
         k.c[6] = k.c[9] = own;
         k.c[7] = vpz;
         k.c[8] = vsz;
@@ -499,7 +504,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
                 if (fld == 0 && active)
                     ctl.trace[(y - a.bord) * (a.sx - 2 * a.bord) + (x - a.bord)] = nw;
             }
-            if (z == src_z) {
+            if (__builtin_expect(z == src_z, 0)) {  // uniform: the source plane only
                 if (col == src_col && active) nw += ctl.src_add;
             }
             bstore32(r_prev, vo8, so_o, nw);

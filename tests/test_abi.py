@@ -15,7 +15,8 @@ HEADERS = [os.path.join(ROOT, "include", "fletcher", h)
 def declared_functions():
     names = set()
     for h in HEADERS:
-        text = open(h).read()
+        with open(h) as f:
+            text = f.read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
             name = m.group(1)

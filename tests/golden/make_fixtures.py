@@ -46,7 +46,9 @@ def parse_logs():
     table = {}
     for path in sorted(glob.glob(os.path.join(LOGS, "*", "*.log"))):
         cur = {}
-        for line in open(path, errors="replace"):
+        with open(path, errors="replace") as fh:
+            lines = fh.readlines()
+        for line in lines:
             m = rx["grid"].search(line)
             if m:
                 cur = {"nx": int(m.group(1)), "ny": int(m.group(2)), "nz": int(m.group(3))}

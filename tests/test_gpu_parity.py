@@ -281,7 +281,8 @@ def test_autotune_keeps_state_and_result(fl, drv, tmp_path):
     d.initialize(shape, D, dt, m)
     d.run(40, isrc, graph_chunk=0)
     assert np.array_equal(d.field("pc"), after_tuned)
-    assert open(cache).read().count("|") >= 4  # cache line written
+    with open(cache) as f:
+        assert f.read().count("|") >= 4  # cache line written
     d.finalize()
     d = drv(variant="fused")
     d.initialize(shape, D, dt, m)

@@ -31,7 +31,8 @@ extras (skipped with --quick), each timed over its whole run with inputs residen
   batched   32 shots on uneven (128..256)^2 x 320 apertures cut from one seeded
             512x512x320 model (absorb 16), 800 steps in 50-step graphs, autotuned
             tiles, FP32, BF16, FP16-recompute and fixed-point-recompute coefficients;
-            receivers every 4 steps
+            receivers every 4 steps, their relative L2 against the FP64 device reference
+            for the first two shots
 """
 import argparse
 import json
@@ -201,6 +202,18 @@ def batched(F, np, precisions, steps, nshots):
     out = {"shots": len(sh), "samples_per_step": samples, "steps": steps, "graph_chunk": 50,
            "dt": dt}
     src = F.sources(dt, steps)
+    # FP64 device reference traces of the first two shots (their own sub-grids, same
+    # source schedule): the batched forms' receiver error
+    nref = min(2, len(sh))
+    ref_tr = []
+    for k in range(nref):
+        r = F.Ref64(sh[k]["shape"], D, dt, sh[k]["model"])
+        try:
+            r.run(steps, sh[k]["isource"], src=src, rec_iz=sh[k]["rec_iz"], rec_every=4,
+                  max_traces=(steps + 3) // 4)
+            ref_tr.append(r.traces())
+        finally:
+            r.destroy()
     for prec in precisions:
         t0 = time.time()
         b = F.Batch(sh, D, dt, precision=prec)
@@ -211,13 +224,15 @@ def batched(F, np, precisions, steps, nshots):
             ms = b.last_ms()
             tr = b.traces(0)
             finite = bool(np.isfinite(tr).all()) and float(np.abs(tr).max()) > 0
+            err = [rel_l2(np, b.traces(k), ref_tr[k]) for k in range(nref)]
         finally:
             b.destroy()
         value = 1e-6 * samples * steps / (ms * 1e-3)
         out[prec] = {"value": round(value, 1), "ms_per_step": round(ms / steps, 4), "tiles": list(tiles),
-                     "finite": finite}
-        log("batched %s: %.0f MS/s (%.1f ms for %d steps; %.1f s incl. upload and tuning)"
-            % (prec, value, ms, steps, time.time() - t0))
+                     "finite": finite,
+                     "traces_rel_l2_vs_fp64": {"shot%d" % k: round(e, 9) for k, e in enumerate(err)}}
+        log("batched %s: %.0f MS/s (%.1f ms for %d steps; %.1f s incl. upload and tuning), traces vs FP64 %s"
+            % (prec, value, ms, steps, time.time() - t0, ["%.2e" % e for e in err]))
     return out
 
 

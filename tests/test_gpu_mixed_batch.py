@@ -55,6 +55,32 @@ def test_precision_error_vs_fp64_oracle(fl, mixed_case, prec, bound):
     assert e_pc <= bound and e_tr <= bound
 
 
+@pytest.mark.parametrize("prec,bound", [("fp16r", 3e-3), ("q16r", 6e-4), ("bf16", 3e-2)])
+def test_anisotropic_precision_error_vs_fp64_oracle(fl, oracle, prec, bound):
+    """The 16-bit forms on a grid with dx != dy != dz (spacing applied per derivative)."""
+    d3 = (12.5, 10.0, 15.0)
+    m = fl.model_gradient((48, 44, 36), 8, seed=77, blob_smin=5.0, blob_smax=12.0, smooth_smin=8.0,
+                          smooth_smax=20.0)
+    shape = m[0].shape
+    dt = fl.cfl_dt(m, 10.0, 0.4)
+    rec = 4 + 8 + 6
+    isrc = fl.index(shape, shape[2] // 2, shape[1] // 2, rec)
+    st = 200
+    ref = oracle.run("der1der1", shape, 4, d3, dt, st, isrc, oracle.precompute(m, "f64"), fp="f64",
+                     rec_iz=rec, rec_every=4)
+    d = fl.Driver(variant="fused", precision=prec)
+    try:
+        d.initialize(shape, d3, dt, m)
+        d.record_traces(rec, 4, st // 4)
+        d.run(st, isrc, graph_chunk=40)
+        e_pc = rel_l2(d.field("pc"), ref["pc"])
+        e_tr = rel_l2(d.traces(), ref["traces"])
+    finally:
+        d.finalize()
+    print("%s anisotropic: relL2 vs FP64 oracle  pc %.3e  traces %.3e" % (prec, e_pc, e_tr))
+    assert e_pc <= bound and e_tr <= bound
+
+
 def _batch_setup(fl, nshots=3, absorb=6):
     from fletcher_petrobras_amd import shots as S
     parent = fl.model_gradient((96, 88, 40), absorb, seed=512, blob_smin=5.0, blob_smax=12.0,
@@ -89,6 +115,30 @@ def test_batch_equals_single_shots(fl, prec, chunk):
         for x, y in zip(got[k], ref):
             assert np.abs(y).max() > 0
             assert np.array_equal(normal_part(x), normal_part(y)), (k, rel_l2(x, y))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "q16r"])
+def test_batch_equals_single_shots_anisotropic(fl, prec):
+    shots, dt = _batch_setup(fl, nshots=2)
+    d3 = (12.5, 10.0, 15.0)
+    dt = dt * 0.8  # the smallest spacing is 10 m here
+    st = 40
+    b = fl.Batch(shots, d3, dt, precision=prec, tiles=(8, 0))
+    try:
+        b.run(st, graph_chunk=10)
+        got = [b.field(k, "pc") for k in range(len(shots))]
+    finally:
+        b.destroy()
+    for k, s in enumerate(shots):
+        d = fl.Driver(variant="fused", precision=prec)
+        try:
+            d.initialize(s["shape"], d3, dt, s["model"])
+            d.run(st, s["isource"], graph_chunk=0)
+            ref = d.field("pc")
+        finally:
+            d.finalize()
+        assert np.abs(ref).max() > 0
+        assert np.array_equal(normal_part(got[k]), normal_part(ref)), (k, rel_l2(got[k], ref))
 
 
 def test_batch_continues_across_runs_and_tiles(fl):

@@ -87,7 +87,7 @@ def tiles_for(d, spec):
         return d.autotune(10)
     ty, zt = (int(v) for v in spec.split(","))
     d.set_tiles(ty, zt)
-    return ty, zt
+    return d.tiles()  # zt 0 resolved to the grid's default chunk depth
 
 
 def dropin(F, np, model, shape, dt, isrc, args, d3=D):

@@ -57,7 +57,7 @@ extern "C" void fl_clear_error(void)
 // ------------------------------------------------------------------ helpers
 // ------------------------------------------------------------------ state
 struct Tiles {
-    int ty = 8, zt = 0;
+    int ty = 12, zt = 0;  // zt 0: chunk_depth's choice for the grid
 };
 
 struct Device {
@@ -120,6 +120,8 @@ static long long pitched_index(long long i, int sx, int sy, int px)
     return (iz * sy + iy) * px + ix;
 }
 
+static int g_cus = 0;  // compute units of the device
+
 static int ensure_stream()
 {
     if (g_stream) return FL_OK;
@@ -127,6 +129,7 @@ static int ensure_stream()
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev < 1) return fail(FL_ERR_HIP, "no HIP device visible");
     HIPCHK(hipSetDevice(0));
+    HIPCHK(hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, 0));
     // default flags: the stream also orders after legacy null-stream work (hipMemcpy etc.)
     HIPCHK(hipStreamCreate(&g_stream));
     return FL_OK;
@@ -213,16 +216,41 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
 }
 
 // ------------------------------------------------------------------ launch plumbing
+// Workgroups of the step kernel resident on one CU: the LDS rings (6 staged planes and 5
+// Dx planes of ty+8 rows) against 160 KB, and the waves per SIMD the kernel is built for
+// (step_waves: 4 for FP32, 6 for the other forms with 8- or 12-row tiles).
+static int resident_per_cu(int ty, int cm)
+{
+    const long long lds = (6ll * (ty + 8) * 80 + 5ll * (ty + 8) * 64) * 4;
+    const int waves = (cm != CM_F32 && (ty == 8 || ty == 12)) ? 6 : 4;
+    return std::max(1, (int)std::min<long long>((160ll << 10) / lds, 4 * waves / ty));
+}
+
+// z chunk depth for `tiles` tile columns per plane over nzu planes: the chunk count that
+// minimises (rounds of resident workgroups) x (planes a workgroup streams, zt + 8).  At
+// 384^3 it gives 32x12x188 for FP32 (768 tiles, three rounds of 256) and 32x12x94 for the
+// other forms (1536 tiles, three rounds of 512), the autotuner's choices.
+static int chunk_depth(long long tiles, int nzu, int ty, int cm, int ztmax)
+{
+    const long long slots = (long long)std::max(g_cus, 1) * resident_per_cu(ty, cm);
+    double best = 1e300;
+    int bz = nzu;
+    for (int nzc = 1; nzc <= 64; nzc++) {
+        const int zt = (nzu + nzc - 1) / nzc;
+        if (zt < 8 && nzc > 1) break;
+        const int n = (nzu + zt - 1) / zt;
+        const long long rounds = (tiles * n + slots - 1) / slots;
+        const double cost = (double)rounds * (zt + 8);
+        if (cost < best * 0.999) { best = cost; bz = zt; }
+    }
+    return std::max(1, std::min(std::min(bz, nzu), ztmax));
+}
+
 static int default_zt(int sx, int sy, int sz, int bord, int ty)
 {
-    const int ntx = tiles_x(sx, bord), nty = (sy - 2 * bord + ty - 1) / ty;
-    const int nzu = sz - 2 * bord;
-    const long long per_plane = (long long)ntx * nty;
-    const long long target = 4096;  // measured best on 256 CUs: ~8 z chunks at 384^3
-    int nzc = (int)std::max<long long>(1, (target + per_plane - 1) / per_plane);
-    int zt = (nzu + nzc - 1) / nzc;
-    zt = std::max(zt, 16);
-    return std::min(std::min(zt, nzu), max_zt(pitch_of(sx), sy));
+    const long long tiles = (long long)tiles_x(sx, bord) * ((sy - 2 * bord + ty - 1) / ty);
+    const int cm = g_variant == FL_VARIANT_FUSED ? D.cm : CM_F32;
+    return chunk_depth(tiles, sz - 2 * bord, ty, cm, max_zt(pitch_of(sx), sy));
 }
 
 static void fill_geom(StepArgs &a, int sx, int sy, int sz, int bord, int ty, int zt)

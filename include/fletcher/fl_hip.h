@@ -61,7 +61,8 @@ int fl_get_precision(void);
 int fl_set_sync(int on);           /* 1 (default): DRIVER_Propagate waits for the step */
 
 /* Tile shape of the z-marching kernel: ty rows (4, 8, 12 or 16) of 32 cells, z chunks of
- * zt planes.  0 selects the built-in default for the grid. */
+ * zt planes.  zt 0 selects the chunk depth that fits whole rounds of resident workgroups
+ * to the grid (the default, with 12-row tiles). */
 int fl_set_tiles(int ty, int zt);
 int fl_get_tiles(int *ty, int *zt);
 

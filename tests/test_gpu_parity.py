@@ -28,7 +28,7 @@ def drv(fl):
     yield make
     for d in made:
         d.finalize()
-    fl.Driver(variant="fused", precision="fp32", sync=True, tiles=(8, 0)).finalize()
+    fl.Driver(variant="fused", precision="fp32", sync=True, tiles=(12, 0)).finalize()
 
 
 def ref_model(fl, form, nx, absorb):

@@ -23,14 +23,14 @@ extras (skipped with --quick), each timed over its whole run with inputs residen
   twopass   the der1der1 two-pass variant on the same grid (104 algorithmic B/sample)
   fp32_graph / mixed   FP32, the FP32 in-kernel recompute (fp32r: FP32 arithmetic, 24-byte
             records), BF16 / FP16 coefficients, the FP16 in-kernel recompute and its
-            16-bit fixed-point variant (q16r) on the same grid, receivers on plane z=28
+            block floating-point variant (q16r) on the same grid, receivers on plane z=28
             every 4 steps, 50-step graph chunks
   rel_l2_vs_fp64   relative L2 (in FP64) of the receiver traces and the final pc against
             the device FP64 reference (fl_ref64_*: the der1der1 loop in double, pinned
             bitwise to the FP64 oracle by the tests) on the same grid and steps
   batched   32 shots on uneven (128..256)^2 x 320 apertures cut from one seeded
             512x512x320 model (absorb 16), 800 steps in 50-step graphs, autotuned
-            tiles, FP32, BF16, FP16-recompute and fixed-point-recompute coefficients;
+            tiles, FP32, BF16, FP16-recompute and block floating-point-recompute coefficients;
             receivers every 4 steps, their relative L2 against the FP64 device reference
             for the first two shots
 """
@@ -240,7 +240,7 @@ def large(F, np, args):
     """The 1024x1024x768 model near HBM capacity: seeded layered model with 12 dipping layers
     (vpz 1500-5000 m/s, eps in [0,0.3], delta in [0,0.2] <= eps, smooth theta and phi), absorb
     32, dx=dy=dz=10 m (array 1096x1096x840), Courant 0.4.  FP32 drop-in, FP32, FP16 and
-    fixed-point recompute, each against the device FP64 reference over the whole grid."""
+    block floating-point recompute, each against the device FP64 reference over the whole grid."""
     d3 = (10.0, 10.0, 10.0)
     t0 = time.time()
     model = F.model_layered((1024, 1024, 768), 32, seed=1024)

@@ -118,8 +118,8 @@ def lib():
         "fl_pack_coefs16": (None, [C.c_int64] + [_f32p] * 6 + [d, i, C.POINTER(vp)]),
         "fl_pack_recompute16": (None, [C.c_int64] + [_f32p] * 6 + [d, C.POINTER(vp)]),
         "fl_pack_recompute32": (None, [C.c_int64] + [_f32p] * 6 + [C.POINTER(vp)]),
-        "fl_recompute_q16_range": (None, [C.c_int64] + [_f32p] * 4 + [d, C.POINTER(d)]),
-        "fl_pack_recompute_q16": (None, [C.c_int64] + [_f32p] * 6 + [d, C.POINTER(d), C.POINTER(vp)]),
+        "fl_recompute_q16_base": (i, [C.c_int64] + [_f32p] * 4 + [d]),
+        "fl_pack_recompute_q16": (None, [C.c_int64] + [_f32p] * 6 + [d, i, C.POINTER(vp)]),
         "fl_extract_window": (None, [_f32p] + [i] * 9 + [_f32p]),
         "fl_rsf_write_field": (i, [C.c_char_p] + [i] * 9 + [f, f, f, _f32p]),
         "fl_rsf_write_traces": (i, [C.c_char_p, i, i, i, f, f, f, _f32p]),

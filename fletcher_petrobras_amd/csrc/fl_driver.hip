@@ -70,7 +70,7 @@ struct Device {
     int lc = 1;             // level holding pc, qc; lv[1 - lc] holds pp, qp
     void *coef = nullptr;   // coefficient records, pitched like the levels
     int cm = CM_F32;
-    float vq = 0, vqs = 0;  // CM_Q16R: fixed-point unit (v2px, v2pz, v2pn); v2sz unit / vq
+    float vq = 0;           // CM_Q16R: 2^ebase
     float *dvx = nullptr, *dvy = nullptr;  // two-pass (pDx, qDx), (pDy, qDy) pairs
     float *trace = nullptr;
     int rec_iz = -1, rec_every = 0, rec_slots = 0;
@@ -162,10 +162,10 @@ extern "C" int fl_bytes_per_cell(void)
 // values (FP32 or the 16-bit forms, or the recompute inputs) in the slot order
 // rec_order32 / rec_order16 / rec_order_r32 / rec_order_r16 (record of
 // coef_count * coef_elem bytes); z slabs of at most 64 MiB per array on the host.
-// vmax: the CM_Q16R full scales (fl_recompute_q16_range over the whole grid).
+// ebase: the CM_Q16R exponent base (fl_recompute_q16_base over the whole grid).
 static int upload_coefs(const float *vpz, const float *vsv, const float *eps, const float *del,
                         const float *phi, const float *theta, float dt, int cm, void *dst, int sx,
-                        int sy, int sz, int px, const double *vmax = nullptr)
+                        int sy, int sz, int px, int ebase = 0)
 {
     const long long sxy = (long long)sx * sy;
     const int slab = std::max(1, (int)std::min<long long>(sz, (64ll << 20) / (sxy * 4)));
@@ -200,7 +200,7 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
                 fl_pack_recompute16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt, c);
             else if (cm == CM_Q16R)
                 fl_pack_recompute_q16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
-                                      vmax, c);
+                                      ebase, c);
             else
                 fl_pack_coefs16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
                                 cm == CM_BF16, c);
@@ -321,7 +321,6 @@ static StepArgs base_args()
     a.dzinv = 1.0f / D.dz;
     a.dt = D.dt;
     a.vq = D.vq;
-    a.vqs = D.vqs;
     const int zt = g_tiles.zt > 0 ? std::min(std::min(g_tiles.zt, D.sz - 2 * D.bord), max_zt(D.px, D.sy))
                                   : default_zt(D.sx, D.sy, D.sz, D.bord, g_tiles.ty);
     fill_geom(a, D.sx, D.sy, D.sz, D.bord, g_tiles.ty, zt);
@@ -489,13 +488,12 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
     }
     const size_t cbytes = (size_t)coef_count(D.cm) * D.np * coef_elem(D.cm);
     if ((rc = dev_alloc(&D.coef, cbytes, &D.bytes))) { free_device(); return rc; }
-    double vmax[2] = {0.0, 0.0};
+    int ebase = 0;
     if (D.cm == CM_Q16R) {
-        fl_recompute_q16_range(D.n, vpz, vsv, eps, del, dt, vmax);
-        D.vq = (float)(vmax[0] / 65535.0);
-        D.vqs = vmax[0] > 0.0 ? (float)(vmax[1] / vmax[0]) : 0.0f;
+        ebase = fl_recompute_q16_base(D.n, vpz, vsv, eps, del, dt);
+        D.vq = std::ldexp(1.0f, ebase);
     }
-    if ((rc = upload_coefs(vpz, vsv, eps, del, phi, theta, dt, D.cm, D.coef, sx, sy, sz, D.px, vmax))) {
+    if ((rc = upload_coefs(vpz, vsv, eps, del, phi, theta, dt, D.cm, D.coef, sx, sy, sz, D.px, ebase))) {
         free_device();
         return rc;
     }

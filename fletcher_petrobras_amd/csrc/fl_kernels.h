@@ -51,7 +51,8 @@ enum CoefMode : int {
     CM_F16 = 2,   // 10 FP16 volumes; v2* pre-multiplied by dt^2
     CM_F16R = 3,  // 4 FP16 velocity factors (times dt^2) and theta, phi as u16 turns
     CM_F32R = 4,  // 4 FP32 velocity factors (precomp.h values) and theta, phi as FP32 turns
-    CM_Q16R = 5,  // 4 velocity factors (times dt^2) in 16-bit fixed point, theta, phi as u16 turns
+    CM_Q16R = 5,  // 4 velocity factors (times dt^2) in per-cell block floating point (15-bit
+                  // mantissas, shared exponent), theta, phi as u16 turns
 };
 
 // Bytes of one cell's coefficient record.
@@ -68,8 +69,8 @@ __host__ __device__ constexpr int rec_bytes(int cm)
 //                       lane's own v2px / v2pn.
 //   BF16 / FP16 (20 B): p lane {0 1 2 3}, q lane {4 5 7 8}, both {6 9}.
 // The recompute inputs are v2px, v2pz, v2sz, v2pn, theta, phi (0..5), stored as
-//   FP16 and fixed-point recompute (12 B): words (v2px, v2pn), (v2pz, v2sz), (theta, phi),
-//                          loaded whole by both lanes;
+//   FP16 and block floating-point recompute (12 B): words (v2px, v2pn), (v2pz, v2sz),
+//                          (theta, phi), loaded whole by both lanes;
 //   FP32 recompute (24 B): p lane {v2px, v2pz, theta}, q lane {v2pn, v2sz, phi}.
 __host__ __device__ constexpr int rec_order32(int s)
 {
@@ -98,7 +99,7 @@ struct StepArgs {
     const void *coef;     // coefficient records: per cell its values, consecutive
     const float *dvx, *dvy;  // TWOPASS: (pDx, qDx) and (pDy, qDy) pair volumes
     float dxxinv, dyyinv, dzzinv, dxinv, dyinv, dzinv, dt;
-    float vq, vqs;        // CM_Q16R: one fixed-point unit of v2px, v2pz, v2pn; of v2sz / vq
+    float vq;             // CM_Q16R: 2^ebase, the unit of the block exponent 0
     int sx, sy, sz, bord, px;
     int ntx, nty, zt, nblk;  // tiles along x, y; z chunk depth; logical blocks
     int xorg;                // x of the first tile column (0 or bord)

@@ -157,9 +157,8 @@ __device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast
 // The coefficients of the own cell in precomp.h order.  For the stored forms c[6] and c[9]
 // both hold the lane's own factor (v2px on p lanes, v2pn on q lanes), the only one the
 // update of the lane's field reads.
-// vqs: CM_Q16R, the v2sz unit over the unit of the other factors.
 template <int CM>
-__device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, float vqs, Coef &k)
+__device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, Coef &k)
 {
     if constexpr (CM == CM_F32) {
         k.c[0] = __builtin_bit_cast(float, from_p(r.w[0]));
@@ -199,10 +198,15 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, float 
             vpz = f16_to_f32((uint16_t)r.w[1]);
             vsz = f16_to_f32((uint16_t)(r.w[1] >> 16));
             t = (float)(fld ? (r.w[2] >> 16) : (r.w[2] & 0xffffu)) * (1.0f / 65536.0f);
-        } else if constexpr (CM == CM_Q16R) {  // as CM_F16R, factors in fixed-point units
-            own = (float)(fld ? (r.w[0] >> 16) : (r.w[0] & 0xffffu));
-            vpz = (float)(r.w[1] & 0xffffu);
-            vsz = (float)(r.w[1] >> 16) * vqs;
+        } else if constexpr (CM == CM_Q16R) {
+            // as CM_F16R, the factors as 15-bit mantissas of the cell's exponent e (bits
+            // 15 and 31 of both words), in units of 2^ebase (applied in the leapfrog)
+            const uint32_t e = ((r.w[0] >> 15) & 1u) | ((r.w[0] >> 30) & 2u) | ((r.w[1] >> 13) & 4u) |
+                               ((r.w[1] >> 28) & 8u);
+            const float sc = __builtin_bit_cast(float, (e + 127u) << 23);  // 2^e
+            own = (float)(fld ? ((r.w[0] >> 16) & 0x7fffu) : (r.w[0] & 0x7fffu)) * sc;
+            vpz = (float)(r.w[1] & 0x7fffu) * sc;
+            vsz = (float)((r.w[1] >> 16) & 0x7fffu) * sc;
             t = (float)(fld ? (r.w[2] >> 16) : (r.w[2] & 0xffffu)) * (1.0f / 65536.0f);
         } else {  // CM_F32R: lane words (own factor, v2pz | v2sz, theta | phi turns)
             own = __builtin_bit_cast(float, r.w[0]);
@@ -280,7 +284,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     const bool active = (x >= a.bord) && (x < a.sx - a.bord) && (y < a.sy - a.bord);
     static_assert(!(ISO && TWOPASS), "two-pass derivative volumes are scaled");
     const float dts = ISO ? a.dxxinv * a.dt : a.dt;  // scale of rhs in the FP32 leapfrog
-    const float rq = ISO ? a.dxxinv * a.vq : a.vq;   // and in the fixed-point form
+    const float rq = ISO ? a.dxxinv * a.vq : a.vq;   // and in the block floating-point form
     const uint32_t pxy = (uint32_t)a.px * (uint32_t)a.sy;  // cells per pitched plane
     const uint32_t pb8 = pxy * 8u, pbr = pxy * (uint32_t)REC;
     const uint32_t col = (uint32_t)(y * a.px + x);
@@ -473,7 +477,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             const float dxz = der1_q<!ISO>(qzx, a.dzinv);
             const float dyz = der1_q<!ISO>(qzy, a.dzinv);
             Coef kc;
-            decode_rec<CM>(kr, fld, a.vqs, kc);
+            decode_rec<CM>(kr, fld, kc);
             // H1, H2 of the own field, association order of original/sample.h:46-53
             float h1 = kc.c[0] * dxx;
             h1 = __builtin_fmaf(kc.c[1], dyy, h1);
@@ -605,7 +609,7 @@ struct ShotDev {
     int sx, sy, sz, px, rec_iz;
     int ntx, nty, item0, nitems;  // tiles along x, y; first work item; item count
     int xorg;                     // x of the first tile column
-    float vq, vqs;                // CM_Q16R units of the shot
+    float vq;                     // CM_Q16R unit of the shot
 };
 
 struct BatchArgs {
@@ -668,7 +672,7 @@ void tti_step_batch(const BatchArgs b)
     a.dvx = a.dvy = nullptr;
     a.dxxinv = b.dxxinv; a.dyyinv = b.dyyinv; a.dzzinv = b.dzzinv;
     a.dxinv = b.dxinv; a.dyinv = b.dyinv; a.dzinv = b.dzinv; a.dt = b.dt;
-    a.vq = sh.vq; a.vqs = sh.vqs;
+    a.vq = sh.vq;
     a.sx = sh.sx; a.sy = sh.sy; a.sz = sh.sz; a.bord = b.bord; a.px = sh.px;
     a.ntx = sh.ntx; a.nty = sh.nty; a.zt = b.zt; a.nblk = sh.nitems; a.xorg = sh.xorg;
     a.rec_iz = sh.rec_iz;

@@ -314,42 +314,76 @@ void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const fl
     }
 }
 
-/* Full scales of the fixed-point velocity factors over n cells (FP64, times dt^2):
- * vmax[0] the largest of v2px, v2pz, v2pn, vmax[1] the largest v2sz. */
-void fl_recompute_q16_range(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
-                            const float *delta, double dt, double *vmax)
+/* Block floating point of the four FP64 velocity factors times dt^2 of one cell: 15-bit
+ * mantissas q[k] and one exponent E shared by the cell, f[k] ~ q[k] * 2^E, with the largest
+ * mantissa in [2^14, 2^15) unless E is clamped to [emin, emin + 15]. */
+static int q16_block(const double f[4], int emin, uint16_t q[4])
 {
-    double m0 = 0.0, m1 = 0.0;
-    const double dt2 = dt * dt;
-#pragma omp parallel for schedule(static) reduction(max : m0, m1)
-    for (int64_t i = 0; i < n; i++) {
-        const double v2pz = (double)vpz[i] * vpz[i];
-        const double a = v2pz * (1.0 + 2.0 * epsilon[i]) * dt2, b = v2pz * dt2,
-                     c = v2pz * (1.0 + 2.0 * delta[i]) * dt2, s = (double)vsv[i] * vsv[i] * dt2;
-        if (a > m0) m0 = a;
-        if (b > m0) m0 = b;
-        if (c > m0) m0 = c;
-        if (s > m1) m1 = s;
+    double m = 0.0;
+    for (int k = 0; k < 4; k++) m = f[k] > m ? f[k] : m;
+    int e = emin;
+    if (m > 0.0) {
+        int ex;
+        frexp(m, &ex);               /* m = fr * 2^ex, fr in [0.5, 1): floor(log2 m) = ex - 1 */
+        e = ex - 1 - 14;
+        if (nearbyint(ldexp(m, -e)) >= 32768.0) e++;
+        if (e < emin) e = emin;
+        if (e > emin + 15) e = emin + 15;
     }
-    vmax[0] = m0;
-    vmax[1] = m1;
+    for (int k = 0; k < 4; k++) {
+        const double u = nearbyint(ldexp(f[k], -e));
+        q[k] = (uint16_t)(u < 0.0 ? 0.0 : (u > 32767.0 ? 32767.0 : u));
+    }
+    return e - emin;
+}
+
+static void q16_factors(int64_t i, const float *vpz, const float *vsv, const float *epsilon,
+                        const float *delta, double dt2, double f[4])
+{
+    const double v2pz = (double)vpz[i] * vpz[i];
+    f[0] = v2pz * (1.0 + 2.0 * epsilon[i]) * dt2;
+    f[1] = v2pz * dt2;
+    f[2] = (double)vsv[i] * vsv[i] * dt2;
+    f[3] = v2pz * (1.0 + 2.0 * delta[i]) * dt2;
+}
+
+int fl_recompute_q16_base(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                          const float *delta, double dt)
+{
+    double m = 0.0;
+    const double dt2 = dt * dt;
+#pragma omp parallel for schedule(static) reduction(max : m)
+    for (int64_t i = 0; i < n; i++) {
+        double f[4];
+        q16_factors(i, vpz, vsv, epsilon, delta, dt2, f);
+        for (int k = 0; k < 4; k++)
+            if (f[k] > m) m = f[k];
+    }
+    if (!(m > 0.0)) return 0;
+    int ex;
+    frexp(m, &ex);
+    int emax = ex - 1 - 14;
+    if (nearbyint(ldexp(m, -emax)) >= 32768.0) emax++;
+    return emax - 15;
 }
 
 void fl_pack_recompute_q16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                            const float *delta, const float *phi, const float *theta, double dt,
-                           const double *vmax, uint16_t *const *out)
+                           int ebase, uint16_t *const *out)
 {
     const double pi = 3.14159265358979323846, dt2 = dt * dt;
-    const double q0 = vmax[0] > 0.0 ? 65535.0 / vmax[0] : 0.0, q1 = vmax[1] > 0.0 ? 65535.0 / vmax[1] : 0.0;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
-        const double v2pz = (double)vpz[i] * vpz[i];
-        const double f[4] = {v2pz * (1.0 + 2.0 * epsilon[i]), v2pz, (double)vsv[i] * vsv[i],
-                             v2pz * (1.0 + 2.0 * delta[i])};
-        for (int k = 0; k < 4; k++) {
-            const double u = nearbyint(f[k] * dt2 * (k == 2 ? q1 : q0));
-            out[k][i] = (uint16_t)(u < 0.0 ? 0.0 : (u > 65535.0 ? 65535.0 : u));
-        }
+        double f[4];
+        uint16_t q[4];
+        q16_factors(i, vpz, vsv, epsilon, delta, dt2, f);
+        const int e = q16_block(f, ebase, q);
+        /* the 4 exponent bits ride in bit 15 of the factors' halves: v2px e0, v2pn e1,
+         * v2pz e2, v2sz e3 (bits 15 and 31 of the record words (v2px, v2pn), (v2pz, v2sz)) */
+        out[0][i] = (uint16_t)(q[0] | ((e & 1) << 15));
+        out[3][i] = (uint16_t)(q[3] | (((e >> 1) & 1) << 15));
+        out[1][i] = (uint16_t)(q[1] | (((e >> 2) & 1) << 15));
+        out[2][i] = (uint16_t)(q[2] | (((e >> 3) & 1) << 15));
         out[4][i] = fl_angle_u16((double)theta[i] / (2.0 * pi));
         out[5][i] = fl_angle_u16((double)phi[i] / (2.0 * pi));
     }

@@ -32,9 +32,9 @@ enum { FL_VARIANT_FUSED = 0, FL_VARIANT_DIRECT = 1, FL_VARIANT_TWOPASS = 2 };
  * store the same 10 volumes (v2* pre-multiplied by dt^2, evaluated in FP64 and
  * rounded once); FP16_RECOMPUTE stores the 4 velocity factors (times dt^2) in FP16
  * and theta, phi as 16-bit fixed point turns, and rebuilds the 6 angular coefficients
- * per cell; Q16_RECOMPUTE stores the same inputs with the velocity factors in 16-bit fixed
- * point (unit: the grid's largest factor / 65535), 16 times finer than FP16 at the top of
- * the range; FP32_RECOMPUTE stores the 4 precomp.h velocity factors bit for bit and
+ * per cell; Q16_RECOMPUTE stores the same inputs with the velocity factors of a cell in
+ * block floating point (15-bit mantissas and one shared exponent), 16 times finer than
+ * FP16 for the cell's largest factor; FP32_RECOMPUTE stores the 4 precomp.h velocity factors bit for bit and
  * theta, phi as FP32 turns, and rebuilds the angular coefficients in FP32 (24 bytes per
  * cell instead of 40).  Wavefields stay FP32.  The recompute forms and the 16-bit forms
  * run with the fused variant. */

@@ -69,14 +69,16 @@ void fl_pack_coefs16(int64_t n, const float *vpz, const float *vsv, const float 
 void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, double dt,
                          uint16_t *const *out);
-/* Inputs of the 16-bit fixed-point recompute: the FP64 velocity factors times dt^2 as
- * round(v * 65535 / vmax), with vmax[0] (v2px, v2pz, v2pn) and vmax[1] (v2sz) from
- * fl_recompute_q16_range over the grid, and theta, phi as fl_angle_u16 turns. */
-void fl_recompute_q16_range(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
-                            const float *delta, double dt, double *vmax);
+/* Inputs of the 16-bit block floating-point recompute: per cell, the four FP64 velocity
+ * factors times dt^2 as 15-bit mantissas with one shared exponent, 2^(ebase + e) with e in
+ * [0, 15] stored one bit per factor in bit 15 (v2px e0, v2pn e1, v2pz e2, v2sz e3); ebase =
+ * fl_recompute_q16_base over the grid (the largest factor gets a mantissa in
+ * [2^14, 2^15) with e = 15).  theta, phi as fl_angle_u16 turns. */
+int fl_recompute_q16_base(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
+                          const float *delta, double dt);
 void fl_pack_recompute_q16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                            const float *delta, const float *phi, const float *theta, double dt,
-                           const double *vmax, uint16_t *const *out);
+                           int ebase, uint16_t *const *out);
 /* Inputs of the in-kernel FP32 recompute: v2px, v2pz, v2sz, v2pn exactly as
  * fl_precompute_f32 (precomp.h:53-58) forms them, and theta, phi in turns reduced to
  * [-1/2, 1/2] (FP64, rounded once to FP32); the six angular coefficients are rebuilt

@@ -248,27 +248,39 @@ def test_pack_recompute32_inputs(fl):
 
 
 def test_pack_recompute_q16_inputs(fl):
-    """Fixed-point recompute inputs: the FP64 velocity factors times dt^2 in units of
-    vmax / 65535, rounded to nearest, with vmax the largest of v2px, v2pz, v2pn over the grid
-    for those three and the largest v2sz for v2sz; the angles as u16 turns (the
-    FP16-recompute encoding)."""
+    """Block floating-point recompute inputs: per cell the FP64 velocity factors times dt^2
+    as 15-bit mantissas of one shared exponent 2^(ebase + e), e in [0, 15] spread over bit
+    15 of the four factors (v2px e0, v2pn e1, v2pz e2, v2sz e3), rounded to nearest; the
+    largest mantissa of a cell in [2^14, 2^15) unless e is clamped; the angles as u16 turns
+    (the FP16-recompute encoding)."""
     m = fl.model_gradient((16, 14, 12), 4, seed=11)
     n = m[0].size
     dt = 8.0e-4
     c64 = fl.precompute(m, "f64")
     dt2 = dt * dt
     want = [c64[6] * dt2, c64[7] * dt2, c64[8] * dt2, c64[9] * dt2]
-    vmax = (C.c_double * 2)()
-    fl.lib().fl_recompute_q16_range(n, m[0], m[1], m[2], m[3], dt, vmax)
-    assert vmax[0] == max(float(want[k].max()) for k in (0, 1, 3))
-    assert vmax[1] == float(want[2].max()) > 0
+    ebase = fl.lib().fl_recompute_q16_base(n, m[0], m[1], m[2], m[3], dt)
+    top = max(float(w.max()) for w in want)
+    assert 2.0 ** 14 <= np.rint(top / 2.0 ** (ebase + 15)) < 2.0 ** 15
     out = [np.empty(m[0].shape, np.uint16) for _ in range(6)]
-    fl.lib().fl_pack_recompute_q16(n, *m, dt, vmax, (C.c_void_p * 6)(*[o.ctypes.data for o in out]))
+    fl.lib().fl_pack_recompute_q16(n, *m, dt, ebase, (C.c_void_p * 6)(*[o.ctypes.data for o in out]))
+    e = ((out[0] >> 15) & 1) | (((out[3] >> 15) & 1) << 1) | (((out[1] >> 15) & 1) << 2) | \
+        (((out[2] >> 15) & 1) << 3)
+    unit = np.ldexp(1.0, ebase + e.astype(np.int64))
+    mant = [(o & 0x7FFF).astype(np.float64) for o in out[:4]]
     for k in range(4):
-        unit = (vmax[1] if k == 2 else vmax[0]) / 65535.0
-        assert np.array_equal(out[k], np.rint(want[k] / unit).astype(np.uint16)), k
-        assert np.abs(out[k] * unit - want[k]).max() <= 0.5 * unit * (1 + 1e-9)
-    assert out[0].max() == 65535 and out[2].max() == 65535
+        assert np.all(np.abs(mant[k] * unit - want[k]) <= 0.5 * unit * (1 + 1e-12)), k
+    big = np.maximum.reduce(mant)
+    live = np.maximum.reduce(want) > 0
+    free = live & (e > 0) & (e < 15)
+    assert free.any() and np.all((big[free] >= 2 ** 14) & (big[free] < 2 ** 15))
+    # relative precision of each cell's largest factor better than 2^-15, unless its exponent
+    # is clamped at the bottom (factors below 2^-15 of the grid's largest), where the unit
+    # 2^ebase bounds the error instead
+    full = live & (e > 0)
+    rel = np.abs(np.maximum.reduce([mant[k] * unit for k in range(4)]) - np.maximum.reduce(want))[full] / \
+        np.maximum.reduce(want)[full]
+    assert full.sum() > 0.9 * live.sum() and rel.max() < 2.0 ** -15
     ref16 = [np.empty(m[0].shape, np.uint16) for _ in range(6)]
     fl.lib().fl_pack_recompute16(n, *m, dt, (C.c_void_p * 6)(*[o.ctypes.data for o in ref16]))
     assert np.array_equal(out[4], ref16[4]) and np.array_equal(out[5], ref16[5])

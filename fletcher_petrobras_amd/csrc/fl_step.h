@@ -200,13 +200,12 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, Coef &
             t = (float)(fld ? (r.w[2] >> 16) : (r.w[2] & 0xffffu)) * (1.0f / 65536.0f);
         } else if constexpr (CM == CM_Q16R) {
             // as CM_F16R, the factors as 15-bit mantissas of the cell's exponent e (bits
-            // 15 and 31 of both words), in units of 2^ebase (applied in the leapfrog)
-            const uint32_t e = ((r.w[0] >> 15) & 1u) | ((r.w[0] >> 30) & 2u) | ((r.w[1] >> 13) & 4u) |
-                               ((r.w[1] >> 28) & 8u);
+            // 30-31 of the two words) in units of 2^ebase, applied in the leapfrog
+            const uint32_t e = (r.w[0] >> 30) | ((r.w[1] >> 28) & 0xcu);
             const float sc = __builtin_bit_cast(float, (e + 127u) << 23);  // 2^e
-            own = (float)(fld ? ((r.w[0] >> 16) & 0x7fffu) : (r.w[0] & 0x7fffu)) * sc;
+            own = (float)((r.w[0] >> (fld ? 15 : 0)) & 0x7fffu) * sc;
             vpz = (float)(r.w[1] & 0x7fffu) * sc;
-            vsz = (float)((r.w[1] >> 16) & 0x7fffu) * sc;
+            vsz = (float)((r.w[1] >> 15) & 0x7fffu) * sc;
             t = (float)(fld ? (r.w[2] >> 16) : (r.w[2] & 0xffffu)) * (1.0f / 65536.0f);
         } else {  // CM_F32R: lane words (own factor, v2pz | v2sz, theta | phi turns)
             own = __builtin_bit_cast(float, r.w[0]);

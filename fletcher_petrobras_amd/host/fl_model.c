@@ -315,24 +315,30 @@ void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const fl
 }
 
 /* Block floating point of the four FP64 velocity factors times dt^2 of one cell: 15-bit
- * mantissas q[k] and one exponent E shared by the cell, f[k] ~ q[k] * 2^E, with the largest
- * mantissa in [2^14, 2^15) unless E is clamped to [emin, emin + 15]. */
-static int q16_block(const double f[4], int emin, uint16_t q[4])
+ * mantissas q * 2^E of one exponent E shared by the cell, the smallest that holds the four,
+ * clamped to [emin, emin + 15]. */
+static int q16_block(const double f[4], int emin, uint32_t q[4])
 {
-    double m = 0.0;
-    for (int k = 0; k < 4; k++) m = f[k] > m ? f[k] : m;
+    double r = 0.0;
+    for (int k = 0; k < 4; k++) r = f[k] / 32767.5 > r ? f[k] / 32767.5 : r;
     int e = emin;
-    if (m > 0.0) {
+    if (r > 0.0) {
         int ex;
-        frexp(m, &ex);               /* m = fr * 2^ex, fr in [0.5, 1): floor(log2 m) = ex - 1 */
-        e = ex - 1 - 14;
-        if (nearbyint(ldexp(m, -e)) >= 32768.0) e++;
+        frexp(r, &ex);  /* r = fr * 2^ex, fr in [0.5, 1): 2^ex is the first power above r */
+        e = ex;
+        while (e > emin) {  /* the smallest E whose rounded mantissas still fit */
+            int fits = 1;
+            for (int k = 0; k < 4; k++)
+                if (nearbyint(ldexp(f[k], -(e - 1))) > 32767.0) fits = 0;
+            if (!fits) break;
+            e--;
+        }
         if (e < emin) e = emin;
         if (e > emin + 15) e = emin + 15;
     }
     for (int k = 0; k < 4; k++) {
         const double u = nearbyint(ldexp(f[k], -e));
-        q[k] = (uint16_t)(u < 0.0 ? 0.0 : (u > 32767.0 ? 32767.0 : u));
+        q[k] = (uint32_t)(u < 0.0 ? 0.0 : (u > 32767.0 ? 32767.0 : u));
     }
     return e - emin;
 }
@@ -350,6 +356,7 @@ static void q16_factors(int64_t i, const float *vpz, const float *vsv, const flo
 int fl_recompute_q16_base(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                           const float *delta, double dt)
 {
+    /* the exponent of the cell that needs the largest one, less 15 */
     double m = 0.0;
     const double dt2 = dt * dt;
 #pragma omp parallel for schedule(static) reduction(max : m)
@@ -357,14 +364,12 @@ int fl_recompute_q16_base(int64_t n, const float *vpz, const float *vsv, const f
         double f[4];
         q16_factors(i, vpz, vsv, epsilon, delta, dt2, f);
         for (int k = 0; k < 4; k++)
-            if (f[k] > m) m = f[k];
+            if (f[k] / 32767.5 > m) m = f[k] / 32767.5;
     }
     if (!(m > 0.0)) return 0;
     int ex;
     frexp(m, &ex);
-    int emax = ex - 1 - 14;
-    if (nearbyint(ldexp(m, -emax)) >= 32768.0) emax++;
-    return emax - 15;
+    return ex - 15;
 }
 
 void fl_pack_recompute_q16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
@@ -375,15 +380,17 @@ void fl_pack_recompute_q16(int64_t n, const float *vpz, const float *vsv, const 
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         double f[4];
-        uint16_t q[4];
+        uint32_t q[4];
         q16_factors(i, vpz, vsv, epsilon, delta, dt2, f);
-        const int e = q16_block(f, ebase, q);
-        /* the 4 exponent bits ride in bit 15 of the factors' halves: v2px e0, v2pn e1,
-         * v2pz e2, v2sz e3 (bits 15 and 31 of the record words (v2px, v2pn), (v2pz, v2sz)) */
-        out[0][i] = (uint16_t)(q[0] | ((e & 1) << 15));
-        out[3][i] = (uint16_t)(q[3] | (((e >> 1) & 1) << 15));
-        out[1][i] = (uint16_t)(q[1] | (((e >> 2) & 1) << 15));
-        out[2][i] = (uint16_t)(q[2] | (((e >> 3) & 1) << 15));
+        const uint32_t e = (uint32_t)q16_block(f, ebase, q);
+        /* record words (v2px, v2pn) and (v2pz, v2sz): mantissas in bits 0-14 and 15-29,
+         * e bits 0-1 in bits 30-31 of the first word, e bits 2-3 in those of the second */
+        const uint32_t w0 = q[0] | (q[3] << 15) | ((e & 3u) << 30);
+        const uint32_t w1 = q[1] | (q[2] << 15) | ((e >> 2) << 30);
+        out[0][i] = (uint16_t)(w0 & 0xffffu);
+        out[3][i] = (uint16_t)(w0 >> 16);
+        out[1][i] = (uint16_t)(w1 & 0xffffu);
+        out[2][i] = (uint16_t)(w1 >> 16);
         out[4][i] = fl_angle_u16((double)theta[i] / (2.0 * pi));
         out[5][i] = fl_angle_u16((double)phi[i] / (2.0 * pi));
     }

@@ -70,10 +70,10 @@ void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const fl
                          const float *delta, const float *phi, const float *theta, double dt,
                          uint16_t *const *out);
 /* Inputs of the 16-bit block floating-point recompute: per cell, the four FP64 velocity
- * factors times dt^2 as 15-bit mantissas with one shared exponent, 2^(ebase + e) with e in
- * [0, 15] stored one bit per factor in bit 15 (v2px e0, v2pn e1, v2pz e2, v2sz e3); ebase =
- * fl_recompute_q16_base over the grid (the largest factor gets a mantissa in
- * [2^14, 2^15) with e = 15).  theta, phi as fl_angle_u16 turns. */
+ * factors times dt^2 as 15-bit mantissas of one shared exponent 2^(ebase + e), e in [0, 15]
+ * the smallest that holds them; as 32-bit words out[0] | out[3] << 16 = v2px | v2pn << 15 |
+ * (e & 3) << 30 and out[1] | out[2] << 16 = v2pz | v2sz << 15 | (e >> 2) << 30.  ebase =
+ * fl_recompute_q16_base over the grid.  theta, phi as fl_angle_u16 turns. */
 int fl_recompute_q16_base(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                           const float *delta, double dt);
 void fl_pack_recompute_q16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,

@@ -248,39 +248,47 @@ def test_pack_recompute32_inputs(fl):
 
 
 def test_pack_recompute_q16_inputs(fl):
-    """Block floating-point recompute inputs: per cell the FP64 velocity factors times dt^2
-    as 15-bit mantissas of one shared exponent 2^(ebase + e), e in [0, 15] spread over bit
-    15 of the four factors (v2px e0, v2pn e1, v2pz e2, v2sz e3), rounded to nearest; the
-    largest mantissa of a cell in [2^14, 2^15) unless e is clamped; the angles as u16 turns
-    (the FP16-recompute encoding)."""
+    """Block floating-point recompute inputs: per cell the FP64 velocity factors times dt^2 as
+    15-bit mantissas of one shared exponent 2^(ebase + e), e in [0, 15] in bits 30-31 of the
+    two record words, rounded to nearest; e the smallest exponent that holds the four unless
+    clamped at 0; the angles as u16 turns (the FP16-recompute encoding)."""
     m = fl.model_gradient((16, 14, 12), 4, seed=11)
     n = m[0].size
     dt = 8.0e-4
     c64 = fl.precompute(m, "f64")
     dt2 = dt * dt
-    want = [c64[6] * dt2, c64[7] * dt2, c64[8] * dt2, c64[9] * dt2]
+    want = [c64[6] * dt2, c64[7] * dt2, c64[8] * dt2, c64[9] * dt2]  # v2px v2pz v2sz v2pn
+    cap = [32767.0] * 4
+    scale = [1.0] * 4
     ebase = fl.lib().fl_recompute_q16_base(n, m[0], m[1], m[2], m[3], dt)
-    top = max(float(w.max()) for w in want)
-    assert 2.0 ** 14 <= np.rint(top / 2.0 ** (ebase + 15)) < 2.0 ** 15
+    # the grid's most demanding cell needs e = 15 exactly
+    need = max(float((want[k] / ((cap[k] + 0.5) * scale[k])).max()) for k in range(4))
+    assert 2.0 ** (ebase + 14) <= need < 2.0 ** (ebase + 15)
     out = [np.empty(m[0].shape, np.uint16) for _ in range(6)]
     fl.lib().fl_pack_recompute_q16(n, *m, dt, ebase, (C.c_void_p * 6)(*[o.ctypes.data for o in out]))
-    e = ((out[0] >> 15) & 1) | (((out[3] >> 15) & 1) << 1) | (((out[1] >> 15) & 1) << 2) | \
-        (((out[2] >> 15) & 1) << 3)
-    unit = np.ldexp(1.0, ebase + e.astype(np.int64))
-    mant = [(o & 0x7FFF).astype(np.float64) for o in out[:4]]
+    w0 = out[0].astype(np.int64) | (out[3].astype(np.int64) << 16)
+    w1 = out[1].astype(np.int64) | (out[2].astype(np.int64) << 16)
+    e = (w0 >> 30) | ((w1 >> 30) << 2)
+    mant = [(w0 & 0x7FFF).astype(np.float64), (w1 & 0x7FFF).astype(np.float64),
+            ((w1 >> 15) & 0x7FFF).astype(np.float64), ((w0 >> 15) & 0x7FFF).astype(np.float64)]
     for k in range(4):
+        unit = np.ldexp(scale[k], ebase + e)
+        # round to nearest: within half a unit, and the closest representable mantissa
         assert np.all(np.abs(mant[k] * unit - want[k]) <= 0.5 * unit * (1 + 1e-12)), k
-    big = np.maximum.reduce(mant)
+        assert np.array_equal(mant[k], np.minimum(np.rint(want[k] / unit), cap[k])), k
+    # e is the smallest exponent that holds the four (one less overflows a mantissa) unless 0
+    lower = np.zeros(e.shape, bool)
+    for k in range(4):
+        lower |= np.rint(want[k] / np.ldexp(scale[k], ebase + e - 1)) > cap[k]
     live = np.maximum.reduce(want) > 0
-    free = live & (e > 0) & (e < 15)
-    assert free.any() and np.all((big[free] >= 2 ** 14) & (big[free] < 2 ** 15))
-    # relative precision of each cell's largest factor better than 2^-15, unless its exponent
-    # is clamped at the bottom (factors below 2^-15 of the grid's largest), where the unit
-    # 2^ebase bounds the error instead
-    full = live & (e > 0)
-    rel = np.abs(np.maximum.reduce([mant[k] * unit for k in range(4)]) - np.maximum.reduce(want))[full] / \
-        np.maximum.reduce(want)[full]
-    assert full.sum() > 0.9 * live.sum() and rel.max() < 2.0 ** -15
+    assert np.all(lower[live & (e > 0)])
+    assert (live & (e > 0)).sum() > 0.9 * live.sum()
+    # the absolute error of each factor within 2^-15 of the cell's largest factor, whose
+    # mantissa is at least 2^14 where e is not clamped
+    top = np.maximum.reduce(want)
+    for k in range(4):
+        err = np.abs(mant[k] * np.ldexp(scale[k], ebase + e) - want[k])
+        assert np.all(err[live & (e > 0)] <= 2.0 ** -15 * top[live & (e > 0)]), k
     ref16 = [np.empty(m[0].shape, np.uint16) for _ in range(6)]
     fl.lib().fl_pack_recompute16(n, *m, dt, (C.c_void_p * 6)(*[o.ctypes.data for o in ref16]))
     assert np.array_equal(out[4], ref16[4]) and np.array_equal(out[5], ref16[5])

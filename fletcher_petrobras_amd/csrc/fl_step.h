@@ -63,28 +63,6 @@ constexpr int prefetch_steps()
 #endif
 }
 
-// Cache policies (0 default, 2 non-temporal).  The previous level is read and written once
-// per step: non-temporal in the single-grid step; the batched step, whose shots are smaller,
-// ran faster with the default.  A coefficient record is non-temporal where one load reads all
-// of it; a record read by two loads (FP32, BF16, FP16) keeps the default, since the hint on
-// either load made those steps slower.  FL_PREV_CPOL, FL_BATCH_PREV_CPOL and FL_COEF_CPOL
-// override at build time.
-#ifndef FL_PREV_CPOL
-#define FL_PREV_CPOL 2
-#endif
-#ifndef FL_BATCH_PREV_CPOL
-#define FL_BATCH_PREV_CPOL 0
-#endif
-template <int CM>
-constexpr int coef_cpol()
-{
-#ifdef FL_COEF_CPOL
-    return FL_COEF_CPOL;
-#else
-    return (CM == CM_F16R || CM == CM_F32R || CM == CM_Q16R) ? 2 : 0;
-#endif
-}
-
 // Lane byte offset beyond every buffer range used here (spans are < 2^31 bytes).
 constexpr uint32_t FL_OOB = 0x80000000u;
 
@@ -142,16 +120,16 @@ template <int CM>
 __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, const RecOff &o, CoefRec<CM> &c)
 {
     if constexpr (CM == CM_F32) {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, o.voff, 0, coef_cpol<CM>());
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, o.voff, 0, 0);
         c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
-        c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff, 16, coef_cpol<CM>());
+        c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff, 16, 0);
     } else if constexpr (CM == CM_F16R || CM == CM_F32R || CM == CM_Q16R) {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b96(r, o.voff, 0, coef_cpol<CM>());
+        const auto a = __builtin_amdgcn_raw_buffer_load_b96(r, o.voff, 0, 0);
         c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2];
     } else {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b64(r, o.voff, 0, coef_cpol<CM>());
+        const auto a = __builtin_amdgcn_raw_buffer_load_b64(r, o.voff, 0, 0);
         c.w[0] = a[0]; c.w[1] = a[1];
-        c.w[2] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff2, 0, coef_cpol<CM>());
+        c.w[2] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff2, 0, 0);
     }
 }
 
@@ -278,7 +256,7 @@ __device__ __forceinline__ void unroll_phases(F &&fn, std::integer_sequence<int,
 // ISO (dx = dy = dz = h): the derivatives stay unscaled sums and 1/h^2 multiplies the
 // right-hand side once, in the leapfrog (the same expression up to rounding: every term
 // of H1, H2 carries 1/h^2).
-template <int TY, int CM, bool TWOPASS, bool ISO, int PCPOL>
+template <int TY, int CM, bool TWOPASS, bool ISO>
 __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl, int tix, int tiy, int zc,
                                           StepLds<TY> &s)
 {
@@ -429,7 +407,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         } else if constexpr (OUT) {
             load_rec<CM>(span_rsrc(cpl, pbr), vor, kr);
             cpl += pbr;
-            vprev = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_prev, vo8, so_o, PCPOL));
+            vprev = __builtin_bit_cast(float, bload32(r_prev, vo8, so_o));
         }
         const bool xw = (XW == 1) || (XW == 2 && i < nz + 4);
         float nx, ny, gxh[NH];
@@ -532,14 +510,14 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             if (__builtin_expect(z == src_z, 0)) {  // uniform: the source plane only
                 if (col == src_col && active) nw += ctl.src_add;
             }
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, nw), r_prev, vo8, so_o, PCPOL);
+            bstore32(r_prev, vo8, so_o, nw);
         }
         if constexpr ((PF & 2) != 0 || (PF & 4) != 0) {
             if ((PF & 2) || pf_next) {
                 constexpr int NS2 = (SET + PD) & 1;
                 load_rec<CM>(span_rsrc(cpl, pbr), vor, kp[NS2]);
                 cpl += pbr;
-                ppv[NS2] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_prev, vo8, (uint32_t)(i + PD - 8) * pb8, PCPOL));
+                ppv[NS2] = __builtin_bit_cast(float, bload32(r_prev, vo8, (uint32_t)(i + PD - 8) * pb8));
             }
         }
         if constexpr (GEN) {
@@ -618,7 +596,7 @@ void tti_step(const StepArgs a)
     if (t >= a.nblk) return;
     const int tix = t % a.ntx;
     const int rest = t / a.ntx;
-    step_tile<TY, CM, TWOPASS, ISO, FL_PREV_CPOL>(a, ctl, tix, rest % a.nty, rest / a.nty, s);
+    step_tile<TY, CM, TWOPASS, ISO>(a, ctl, tix, rest % a.nty, rest / a.nty, s);
 }
 
 // Shot of a batched run (device copy of the host descriptor).
@@ -710,7 +688,7 @@ void tti_step_batch(const BatchArgs b)
     const int l = t - sh.item0;
     const int tix = l % sh.ntx;
     const int rest = l / sh.ntx;
-    step_tile<TY, CM, false, ISO, FL_BATCH_PREV_CPOL>(a, ctl, tix, rest % sh.nty, rest / sh.nty, s);
+    step_tile<TY, CM, false, ISO>(a, ctl, tix, rest % sh.nty, rest / sh.nty, s);
 }
 
 }  // namespace fl

@@ -83,11 +83,15 @@ def cpu_baseline(model, shape, dt, isrc, steps, threads):
 
 
 def tiles_for(d, spec):
+    """(ty, zt, stream hints): the autotuner's choice for 'auto', else 'ty,zt[,hints]'."""
     if spec == "auto":
-        return d.autotune(10)
-    ty, zt = (int(v) for v in spec.split(","))
-    d.set_tiles(ty, zt)
-    return d.tiles()  # zt 0 resolved to the grid's default chunk depth
+        ty, zt = d.autotune(10)
+        return ty, zt, int(d.stream_hints())
+    v = [int(x) for x in spec.split(",")]
+    d.set_tiles(v[0], v[1])
+    d.set_stream_hints(v[2] if len(v) > 2 else 0)
+    ty, zt = d.tiles()  # zt 0 resolved to the grid's default chunk depth
+    return ty, zt, int(d.stream_hints())
 
 
 def dropin(F, np, model, shape, dt, isrc, args, d3=D):
@@ -97,8 +101,8 @@ def dropin(F, np, model, shape, dt, isrc, args, d3=D):
     t0 = time.time()
     d.initialize(shape, d3, dt, model)
     log("initialize (precompute + upload) %.1f s, device %s" % (time.time() - t0, F.device_name()))
-    ty, zt = tiles_for(d, args.tiles)
-    log("tiles ty=%d zt=%d" % (ty, zt))
+    ty, zt, hints = tiles_for(d, args.tiles)
+    log("tiles ty=%d zt=%d stream hints %d" % (ty, zt, hints))
     bpc = d.bytes_per_cell()
     for it in range(1, args.warmup + 1):
         d.insert_source(it - 1, isrc, F.source(dt, it - 1))
@@ -123,7 +127,7 @@ def dropin(F, np, model, shape, dt, isrc, args, d3=D):
     d.finalize()
     avg_ms = kms / max(kcount, 1)
     r = {"value": 1e-6 * samples * args.steps / (region_ms * 1e-3), "region_ms": region_ms,
-         "avg_kernel_ms": avg_ms, "launches": kcount, "bpc": bpc, "tiles": (ty, zt),
+         "avg_kernel_ms": avg_ms, "launches": kcount, "bpc": bpc, "tiles": (ty, zt), "hints": hints,
          "achieved": bpc * samples / (avg_ms * 1e-3) / 1e9, "finite": finite, "pc": pc}
     log("drop-in %s/%s: region %.1f ms (host wall %.1f ms), kernel avg %.4f ms over %d, %.0f GB/s, "
         "%.0f MS/s, finite=%s" % (args.variant, args.precision, region_ms, wall * 1e3, avg_ms, kcount,
@@ -161,7 +165,7 @@ def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, 
     samples = (shape[0] - 8) * (shape[1] - 8) * (shape[2] - 8)
     d = F.Driver(variant=variant, precision=precision, sync=False)
     d.initialize(shape, d3, dt, model)
-    ty, zt = tiles_for(d, tiles)
+    ty, zt, hints = tiles_for(d, tiles)
     bpc = d.bytes_per_cell()
     d.record_traces(rec_iz, 4, (steps + 3) // 4)
     src = F.sources(dt, steps)
@@ -177,7 +181,7 @@ def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, 
     value = 1e-6 * samples * (steps - 2) / (region_ms * 1e-3)
     r = {"value": round(value, 1), "ms_per_step": round(region_ms / (steps - 2), 4),
          "bytes_per_sample": bpc, "achieved_GBs": round(bpc * value * 1e-3, 1),
-         "frac": round(bpc * value * 1e-3 / HBM_PEAK_GBS, 4), "tiles": [ty, zt], "finite": finite}
+         "frac": round(bpc * value * 1e-3 / HBM_PEAK_GBS, 4), "tiles": [ty, zt], "stream_hints": hints, "finite": finite}
     if ref is not None:
         r["rel_l2_vs_fp64"] = {"traces": round(rel_l2(np, tr, ref["traces"]), 9),
                                "pc": round(rel_l2(np, pc, ref["pc_steps"]), 9)}
@@ -268,8 +272,9 @@ def large(F, np, args):
         "vs_baseline": None, "dtype": "f32",
         "data": "synthetic seeded layered TTI model (12 dipping layers, smooth theta/phi, reference random boundary)",
         "config": {"workload": "TTI 1024x1024x768 interior, absorb 32, array %dx%dx%d, dx=10 m, dt=%.6g s, "
-                               "fused variant, fp32 coefficients, tiles 32x%dx%d"
-                               % (shape[2], shape[1], shape[0], dt, main_r["tiles"][0], main_r["tiles"][1]),
+                               "fused variant, fp32 coefficients, tiles 32x%dx%d, stream hints %s"
+                               % (shape[2], shape[1], shape[0], dt, main_r["tiles"][0], main_r["tiles"][1],
+                                  "on" if main_r["hints"] else "off"),
                    "samples_per_step": samples, "bytes_per_sample": main_r["bpc"]},
         "roofline": {"bound": "hbm", "achieved": round(main_r["achieved"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_r["achieved"] / HBM_PEAK_GBS, 4), "traffic": None},
@@ -288,7 +293,7 @@ def main():
     ap.add_argument("--absorb", type=int, default=16)
     ap.add_argument("--variant", default="fused", choices=["fused", "direct", "twopass"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r"])
-    ap.add_argument("--tiles", default="auto", help="ty,zt or 'auto' (autotuner)")
+    ap.add_argument("--tiles", default="auto", help="ty,zt[,stream hints 0/1] or 'auto' (autotuner)")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline line only, no extras")
@@ -372,9 +377,9 @@ def main():
         "data": "synthetic seeded heterogeneous TTI model (vpz gradient + blobs, smooth eps/delta/theta/phi, "
                 "reference random boundary)",
         "config": {"workload": "TTI %d^3 interior, absorb %d, array %dx%dx%d, dx=12.5 m, dt=%.6g s, "
-                               "%s variant, %s coefficients, tiles 32x%dx%d"
+                               "%s variant, %s coefficients, tiles 32x%dx%d, stream hints %s"
                                % (n, args.absorb, shape[2], shape[1], shape[0], dt, args.variant,
-                                  args.precision, ty, zt),
+                                  args.precision, ty, zt, "on" if main_r["hints"] else "off"),
                    "samples_per_step": samples, "bytes_per_sample": bpc},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(tag)},

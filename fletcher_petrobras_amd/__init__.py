@@ -67,6 +67,8 @@ def lib():
         "fl_get_precision": (i, []),
         "fl_set_sync": (i, [i]),
         "fl_set_tiles": (i, [i, i]),
+        "fl_set_stream_hints": (i, [i]),
+        "fl_get_stream_hints": (i, []),
         "fl_get_tiles": (i, [C.POINTER(i), C.POINTER(i)]),
         "fl_autotune": (i, [i, C.c_char_p]),
         "fl_get_field": (i, [i, _f32p]),
@@ -370,6 +372,12 @@ class Driver:
         a, b = C.c_int(), C.c_int()
         self.L.fl_get_tiles(C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def set_stream_hints(self, on):
+        _check(self.L.fl_set_stream_hints(1 if on else 0))
+
+    def stream_hints(self):
+        return bool(self.L.fl_get_stream_hints())
 
     def autotune(self, steps=5, cache=None):
         _check(self.L.fl_autotune(steps, cache.encode() if cache else None))

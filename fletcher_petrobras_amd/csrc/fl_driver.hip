@@ -58,6 +58,7 @@ extern "C" void fl_clear_error(void)
 // ------------------------------------------------------------------ state
 struct Tiles {
     int ty = 12, zt = 0;  // zt 0: chunk_depth's choice for the grid
+    int hints = 0;        // streaming cache hints in the fused step (fl_set_stream_hints)
 };
 
 struct Device {
@@ -264,11 +265,11 @@ static void fill_geom(StepArgs &a, int sx, int sy, int sz, int bord, int ty, int
     a.nblk = a.ntx * a.nty * nzc;
 }
 
-template <int TY, int CM, bool TP, bool ISO>
+template <int TY, int CM, bool TP, bool ISO, bool HINTS = false>
 static void launch_zm(const StepArgs &a, hipStream_t s)
 {
     const int g = (a.nblk + 7) / 8 * 8;
-    hipLaunchKernelGGL((tti_step<TY, CM, TP, ISO>), dim3(g), dim3(64, TY), 0, s, a);
+    hipLaunchKernelGGL((tti_step<TY, CM, TP, ISO, HINTS>), dim3(g), dim3(64, TY), 0, s, a);
 }
 
 // Equal spacings select the kernels that scale the right-hand side once (ISO).
@@ -277,10 +278,16 @@ static bool iso_spacing(float dx, float dy, float dz) { return dx == dy && dy ==
 static int launch_steps(const StepArgs &a, int ty, int cm, bool tp, hipStream_t s)
 {
     const bool iso = iso_spacing(D.dx, D.dy, D.dz);
+    const bool hints = g_tiles.hints != 0;
 #define FL_ZM1(TY, CM)                                                     \
     case CM:                                                               \
-        if (iso) launch_zm<TY, CM, false, true>(a, s);                     \
-        else launch_zm<TY, CM, false, false>(a, s);                        \
+        if (iso) {                                                         \
+            if (hints) launch_zm<TY, CM, false, true, true>(a, s);         \
+            else launch_zm<TY, CM, false, true>(a, s);                     \
+        } else {                                                           \
+            if (hints) launch_zm<TY, CM, false, false, true>(a, s);        \
+            else launch_zm<TY, CM, false, false>(a, s);                    \
+        }                                                                  \
         return FL_OK;
 #define FL_ZM(TY)                                                          \
     if (ty == TY) {                                                        \
@@ -639,6 +646,15 @@ extern "C" int fl_set_tiles(int ty, int zt)
     return FL_OK;
 }
 
+extern "C" int fl_set_stream_hints(int on)
+{
+    if (on != 0 && on != 1) return fail(FL_ERR_ARG, "stream hints must be 0 or 1 (got %d)", on);
+    g_tiles.hints = on;
+    return FL_OK;
+}
+
+extern "C" int fl_get_stream_hints(void) { return g_tiles.hints; }
+
 extern "C" int fl_get_tiles(int *ty, int *zt)
 {
     if (ty) *ty = g_tiles.ty;
@@ -851,7 +867,8 @@ extern "C" int fl_device_name(char *buf, int len)
 extern "C" long long fl_device_bytes(void) { return D.bytes; }
 
 // ------------------------------------------------------------------ autotuner
-static bool cache_lookup(const char *path, const std::string &key, int *ty, int *zt)
+static bool cache_lookup(const char *path, const std::string &key, int *ty, int *zt,
+                         int *hints = nullptr)
 {
     if (!path) return false;
     FILE *f = fopen(path, "r");
@@ -860,9 +877,12 @@ static bool cache_lookup(const char *path, const std::string &key, int *ty, int 
     bool found = false;
     while (fgets(line, sizeof(line), f)) {
         char k[400];
-        int a, b;
-        if (sscanf(line, "%399s %d %d", k, &a, &b) == 3 && key == k) {
+        int a, b, h = 0;
+        double ms;
+        const int n = sscanf(line, "%399s %d %d %lf %d", k, &a, &b, &ms, &h);
+        if (n >= 3 && key == k) {
             *ty = a; *zt = b;
+            if (hints) *hints = n == 5 ? h : 0;
             found = true;  // last entry wins
         }
     }
@@ -870,12 +890,13 @@ static bool cache_lookup(const char *path, const std::string &key, int *ty, int 
     return found;
 }
 
-static void cache_store(const char *path, const std::string &key, int ty, int zt, double ms)
+static void cache_store(const char *path, const std::string &key, int ty, int zt, double ms,
+                        int hints = 0)
 {
     if (!path) return;
     FILE *f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%s %d %d %.4f\n", key.c_str(), ty, zt, ms);
+    fprintf(f, "%s %d %d %.4f %d\n", key.c_str(), ty, zt, ms, hints);
     fclose(f);
 }
 
@@ -909,8 +930,11 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
     if (!D.init) return fail(FL_ERR_STATE, "fl_autotune before DRIVER_Initialize");
     if (g_variant == FL_VARIANT_DIRECT) return FL_OK;  // no tile parameters
     const std::string key = tune_key("grid", D.sx, D.sy, D.sz, D.cm);
-    int ty, zt;
-    if (cache_lookup(cache_path, key, &ty, &zt)) return fl_set_tiles(ty, zt);
+    int ty, zt, hints = 0;
+    if (cache_lookup(cache_path, key, &ty, &zt, &hints)) {
+        g_tiles.hints = g_variant == FL_VARIANT_FUSED ? hints : 0;
+        return fl_set_tiles(ty, zt);
+    }
     steps = std::max(1, steps);
     // keep the fields: tuning steps run on them and are undone afterwards
     float *save = nullptr;
@@ -928,12 +952,16 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
     double best = 1e30;
     Tiles bt = keep;
     // two passes over the candidates, the fastest single measurement wins: one pass
-    // leaves the early candidates timed while the clocks still ramp
+    // leaves the early candidates timed while the clocks still ramp.  The fused step is
+    // also timed with and without the streaming hints.
+    const int nh = g_variant == FL_VARIANT_FUSED ? 2 : 1;
     for (int pass = 0; pass < 2 && rc == FL_OK; pass++)
+    for (int ch = 0; ch < nh && rc == FL_OK; ch++)
     for (int cty : {4, 8, 12, 16}) {
         for (int czt : zt_candidates(std::min(D.sz - 2 * D.bord, max_zt(D.px, D.sy)))) {
             g_tiles.ty = cty;
             g_tiles.zt = czt;
+            g_tiles.hints = ch;
             StepArgs a = base_args();
             a.trace = nullptr;
             rc = launch_step(a, 0, g_stream);  // warm-up
@@ -969,7 +997,7 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
         return rc;
     }
     g_tiles = bt;
-    cache_store(cache_path, key, bt.ty, bt.zt, best / steps);
+    cache_store(cache_path, key, bt.ty, bt.zt, best / steps, bt.hints);
     return FL_OK;
 }
 

@@ -63,6 +63,18 @@ constexpr int prefetch_steps()
 #endif
 }
 
+// Streaming hints (PCPOL 2, off 0): the previous level, read and written once per step, and
+// the coefficient records that one load reads whole (the recompute forms) are loaded and
+// stored non-temporal.  Records read by two loads (FP32, BF16, FP16) keep the default
+// policy: the hint on either load made those steps slower.  Whether the hints pay depends
+// on the grid (faster on a 384^3 array, slower on a 1096x1096x840 one), so the host
+// selects them per grid.
+template <int CM, int PCPOL>
+constexpr int coef_cpol()
+{
+    return (PCPOL && (CM == CM_F16R || CM == CM_F32R || CM == CM_Q16R)) ? 2 : 0;
+}
+
 // Lane byte offset beyond every buffer range used here (spans are < 2^31 bytes).
 constexpr uint32_t FL_OOB = 0x80000000u;
 
@@ -116,7 +128,7 @@ __device__ __forceinline__ RecOff rec_offsets(uint32_t col, int fld, bool active
     return o;
 }
 
-template <int CM>
+template <int CM, int PCPOL>
 __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, const RecOff &o, CoefRec<CM> &c)
 {
     if constexpr (CM == CM_F32) {
@@ -124,7 +136,7 @@ __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, const RecOff 
         c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
         c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff, 16, 0);
     } else if constexpr (CM == CM_F16R || CM == CM_F32R || CM == CM_Q16R) {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b96(r, o.voff, 0, 0);
+        const auto a = __builtin_amdgcn_raw_buffer_load_b96(r, o.voff, 0, coef_cpol<CM, PCPOL>());
         c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2];
     } else {
         const auto a = __builtin_amdgcn_raw_buffer_load_b64(r, o.voff, 0, 0);
@@ -256,7 +268,7 @@ __device__ __forceinline__ void unroll_phases(F &&fn, std::integer_sequence<int,
 // ISO (dx = dy = dz = h): the derivatives stay unscaled sums and 1/h^2 multiplies the
 // right-hand side once, in the leapfrog (the same expression up to rounding: every term
 // of H1, H2 carries 1/h^2).
-template <int TY, int CM, bool TWOPASS, bool ISO>
+template <int TY, int CM, bool TWOPASS, bool ISO, int PCPOL>
 __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl, int tix, int tiy, int zc,
                                           StepLds<TY> &s)
 {
@@ -405,9 +417,9 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             kr = kp[SET];
             vprev = ppv[SET];
         } else if constexpr (OUT) {
-            load_rec<CM>(span_rsrc(cpl, pbr), vor, kr);
+            load_rec<CM, PCPOL>(span_rsrc(cpl, pbr), vor, kr);
             cpl += pbr;
-            vprev = __builtin_bit_cast(float, bload32(r_prev, vo8, so_o));
+            vprev = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_prev, vo8, so_o, PCPOL));
         }
         const bool xw = (XW == 1) || (XW == 2 && i < nz + 4);
         float nx, ny, gxh[NH];
@@ -510,14 +522,14 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             if (__builtin_expect(z == src_z, 0)) {  // uniform: the source plane only
                 if (col == src_col && active) nw += ctl.src_add;
             }
-            bstore32(r_prev, vo8, so_o, nw);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, nw), r_prev, vo8, so_o, PCPOL);
         }
         if constexpr ((PF & 2) != 0 || (PF & 4) != 0) {
             if ((PF & 2) || pf_next) {
                 constexpr int NS2 = (SET + PD) & 1;
-                load_rec<CM>(span_rsrc(cpl, pbr), vor, kp[NS2]);
+                load_rec<CM, PCPOL>(span_rsrc(cpl, pbr), vor, kp[NS2]);
                 cpl += pbr;
-                ppv[NS2] = __builtin_bit_cast(float, bload32(r_prev, vo8, (uint32_t)(i + PD - 8) * pb8));
+                ppv[NS2] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_prev, vo8, (uint32_t)(i + PD - 8) * pb8, PCPOL));
             }
         }
         if constexpr (GEN) {
@@ -584,7 +596,7 @@ constexpr int step_waves()
 #endif
 }
 
-template <int TY, int CM, bool TWOPASS, bool ISO>
+template <int TY, int CM, bool TWOPASS, bool ISO, bool HINTS>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(step_waves<TY, CM>())))
 void tti_step(const StepArgs a)
 {
@@ -596,7 +608,7 @@ void tti_step(const StepArgs a)
     if (t >= a.nblk) return;
     const int tix = t % a.ntx;
     const int rest = t / a.ntx;
-    step_tile<TY, CM, TWOPASS, ISO>(a, ctl, tix, rest % a.nty, rest / a.nty, s);
+    step_tile<TY, CM, TWOPASS, ISO, HINTS ? 2 : 0>(a, ctl, tix, rest % a.nty, rest / a.nty, s);
 }
 
 // Shot of a batched run (device copy of the host descriptor).
@@ -688,7 +700,7 @@ void tti_step_batch(const BatchArgs b)
     const int l = t - sh.item0;
     const int tix = l % sh.ntx;
     const int rest = l / sh.ntx;
-    step_tile<TY, CM, false, ISO>(a, ctl, tix, rest % sh.nty, rest / sh.nty, s);
+    step_tile<TY, CM, false, ISO, 0>(a, ctl, tix, rest % sh.nty, rest / sh.nty, s);
 }
 
 }  // namespace fl

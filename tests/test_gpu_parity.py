@@ -28,7 +28,9 @@ def drv(fl):
     yield make
     for d in made:
         d.finalize()
-    fl.Driver(variant="fused", precision="fp32", sync=True, tiles=(12, 0)).finalize()
+    z = fl.Driver(variant="fused", precision="fp32", sync=True, tiles=(12, 0))
+    z.set_stream_hints(False)
+    z.finalize()
 
 
 def ref_model(fl, form, nx, absorb):
@@ -210,6 +212,28 @@ def test_tile_shapes_are_bitwise_identical(fl, drv, tiles):
         assert np.array_equal(normal_part(out[0][k]), normal_part(out[1][k]))
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp32r", "bf16", "fp16r", "q16r"])
+def test_stream_hints_are_bitwise_identical(fl, drv, prec):
+    """The streaming cache hints change only the cache policy of the previous-level and
+    coefficient accesses: the fields after 40 steps are the same bits with and without."""
+    m, dt = hetero(fl, (60, 33, 41), 6)
+    shape = m[0].shape
+    isrc = fl.index(shape, 20, 17, 15)
+    out = []
+    for h in (False, True):
+        d = drv(variant="fused", precision=prec, tiles=(12, 0))
+        d.set_stream_hints(h)
+        assert d.stream_hints() == h
+        d.initialize(shape, D, dt, m)
+        d.run(40, isrc, graph_chunk=10)
+        out.append((d.field("pc"), d.field("qc")))
+        d.finalize()
+    for k in (0, 1):
+        assert np.array_equal(out[0][k], out[1][k])
+    assert fl.lib().fl_set_stream_hints(2) == -1  # FL_ERR_ARG
+    fl.lib().fl_clear_error()
+
+
 def test_twopass_tiles_bitwise(fl, drv):
     m, dt = hetero(fl, (40, 30, 28), 6)
     shape = m[0].shape
@@ -295,12 +319,14 @@ def test_autotune_keeps_state_and_result(fl, drv, tmp_path):
     before = d.field("pc")
     cache = str(tmp_path / "tiles.txt")
     ty, zt = d.autotune(2, cache)
+    hints = d.stream_hints()
     assert ty in (4, 8, 12, 16) and zt > 0
     assert np.array_equal(d.field("pc"), before)
     d.run(20, isrc, it0=20, graph_chunk=0)
     after_tuned = d.field("pc")
     d.finalize()
     d = drv(variant="fused", tiles=(8, 0))
+    d.set_stream_hints(not hints)
     d.initialize(shape, D, dt, m)
     d.run(40, isrc, graph_chunk=0)
     assert np.array_equal(d.field("pc"), after_tuned)
@@ -310,6 +336,7 @@ def test_autotune_keeps_state_and_result(fl, drv, tmp_path):
     d = drv(variant="fused")
     d.initialize(shape, D, dt, m)
     assert d.autotune(2, cache) == (ty, zt)  # served from the cache
+    assert d.stream_hints() == hints
 
 
 def test_initialize_with_host_fields_and_update_pointers(fl, drv):

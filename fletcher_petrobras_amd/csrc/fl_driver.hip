@@ -511,8 +511,8 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
 
 // Selection through the environment for front-ends that only call driver.h (the
 // reference's main.c/model.c): FL_VARIANT = fused|direct|twopass, FL_PRECISION =
-// fp32|bf16|fp16|fp16r|fp32r|q16r, applied at DRIVER_Initialize when set; FL_TUNE = cache file, the
-// autotuner runs (or reads the cache) right after it.
+// fp32|bf16|fp16|fp16r|fp32r|q16r, FL_STREAM_HINTS = 0|1, applied at DRIVER_Initialize when set;
+// FL_TUNE = cache file, the autotuner runs (or reads the cache) right after it.
 static int env_pick(const char *name, const char *const *opts, int n)
 {
     const char *v = getenv(name);
@@ -532,13 +532,16 @@ extern "C" void DRIVER_Initialize(const int sx, const int sy, const int sz, cons
 {
     static const char *const variants[] = {"fused", "direct", "twopass"};
     static const char *const precs[] = {"fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r"};
+    static const char *const onoff[] = {"0", "1"};
     const int v = env_pick("FL_VARIANT", variants, 3), p = env_pick("FL_PRECISION", precs, 6);
-    if (v == -2 || p == -2) {
-        fail(FL_ERR_ARG, "FL_VARIANT / FL_PRECISION hold an unknown value");
+    const int h = env_pick("FL_STREAM_HINTS", onoff, 2);
+    if (v == -2 || p == -2 || h == -2) {
+        fail(FL_ERR_ARG, "FL_VARIANT / FL_PRECISION / FL_STREAM_HINTS hold an unknown value");
         return;
     }
     if (v >= 0) g_variant = v;
     if (p >= 0) g_precision = p;
+    if (h >= 0) g_tiles.hints = h;
     if (initialize(sx, sy, sz, bord, dx, dy, dz, dt, vpz, vsv, epsilon, delta, phi, theta, pp, pc, qp, qc) != FL_OK)
         return;
     fl_clear_error();

@@ -87,10 +87,6 @@ __device__ __forceinline__ uint32_t bload32(__amdgpu_buffer_rsrc_t r, uint32_t v
 {
     return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
 }
-__device__ __forceinline__ void bstore32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
-}
 
 // Coefficient record as loaded by one lane: FP32 its half of the record (5 values, 20 B);
 // BF16 / FP16 its half (2 words) and the shared word of v2px, v2pn; FP32 recompute its

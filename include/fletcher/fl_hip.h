@@ -53,7 +53,8 @@ void fl_clear_error(void);
 
 /* Front-ends that call only driver.h select with the environment instead, read by
  * DRIVER_Initialize: FL_VARIANT=fused|direct|twopass, FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r|q16r,
- * FL_TUNE=<cache file> (autotune right after initialisation). */
+ * FL_STREAM_HINTS=0|1 (fl_set_stream_hints), FL_TUNE=<cache file> (autotune right after
+ * initialisation). */
 int fl_set_variant(int variant);   /* takes effect at the next step */
 int fl_get_variant(void);
 int fl_set_precision(int precision); /* used by the next DRIVER_Initialize */

@@ -35,6 +35,20 @@ def test_reference_front_end_drives_the_device(oracle, cpu_ref, variant, monkeyp
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp32r"])
+def test_reference_front_end_stream_hints(oracle, cpu_ref, prec, monkeypatch):
+    """FL_STREAM_HINTS=1 then 0: the same pc bits both ways, within 1e-5 of the reference
+    OpenMP build."""
+    monkeypatch.setenv("FL_VARIANT", "fused")
+    monkeypatch.setenv("FL_PRECISION", prec)
+    got = []
+    for h in ("1", "0"):
+        monkeypatch.setenv("FL_STREAM_HINTS", h)
+        got.append(oracle.Ref("dropin").main(ARGS)["pc"])
+    assert (got[0] == got[1]).all()
+    assert rel_l2(got[0], cpu_ref["pc"]) <= 1e-5
+
+
 def test_reference_front_end_fp32_recompute(oracle, cpu_ref, monkeypatch):
     """FL_PRECISION=fp32r: 24-byte records, angular coefficients rebuilt in FP32 on the
     device; the same 1e-5 bar against the reference OpenMP build."""

@@ -58,7 +58,7 @@ extern "C" void fl_clear_error(void)
 // ------------------------------------------------------------------ state
 struct Tiles {
     int ty = 12, zt = 0;  // zt 0: chunk_depth's choice for the grid
-    int hints = 0;        // streaming cache hints in the fused step (fl_set_stream_hints)
+    int hints = 0;        // streaming cache hints of the z-marching step (fl_set_stream_hints)
 };
 
 struct Device {
@@ -291,7 +291,11 @@ static int launch_steps(const StepArgs &a, int ty, int cm, bool tp, hipStream_t 
         return FL_OK;
 #define FL_ZM(TY)                                                          \
     if (ty == TY) {                                                        \
-        if (tp) { launch_zm<TY, CM_F32, true, false>(a, s); return FL_OK; } \
+        if (tp) {                                                          \
+            if (hints) launch_zm<TY, CM_F32, true, false, true>(a, s);     \
+            else launch_zm<TY, CM_F32, true, false>(a, s);                 \
+            return FL_OK;                                                  \
+        }                                                                  \
         switch (cm) {                                                      \
             FL_ZM1(TY, CM_F32)                                             \
             FL_ZM1(TY, CM_BF16)                                            \
@@ -935,7 +939,7 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
     const std::string key = tune_key("grid", D.sx, D.sy, D.sz, D.cm);
     int ty, zt, hints = 0;
     if (cache_lookup(cache_path, key, &ty, &zt, &hints)) {
-        g_tiles.hints = g_variant == FL_VARIANT_FUSED ? hints : 0;
+        g_tiles.hints = hints;
         return fl_set_tiles(ty, zt);
     }
     steps = std::max(1, steps);
@@ -955,9 +959,9 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
     double best = 1e30;
     Tiles bt = keep;
     // two passes over the candidates, the fastest single measurement wins: one pass
-    // leaves the early candidates timed while the clocks still ramp.  The fused step is
-    // also timed with and without the streaming hints.
-    const int nh = g_variant == FL_VARIANT_FUSED ? 2 : 1;
+    // leaves the early candidates timed while the clocks still ramp.  Each is timed with
+    // and without the streaming hints.
+    const int nh = 2;
     for (int pass = 0; pass < 2 && rc == FL_OK; pass++)
     for (int ch = 0; ch < nh && rc == FL_OK; ch++)
     for (int cty : {4, 8, 12, 16}) {

@@ -67,14 +67,14 @@ int fl_set_sync(int on);           /* 1 (default): DRIVER_Propagate waits for th
 int fl_set_tiles(int ty, int zt);
 int fl_get_tiles(int *ty, int *zt);
 
-/* Streaming cache hints of the fused step: 1 loads and stores the previous level, and loads
- * the recompute forms' coefficient records, non-temporal; 0 (the default) uses the default
- * policy.  Results are identical either way; which is faster depends on the grid, and
- * fl_autotune times both. */
+/* Streaming cache hints of the z-marching step (fused and two-pass variants): 1 loads and
+ * stores the previous level, and loads the recompute forms' coefficient records,
+ * non-temporal; 0 (the default) uses the default policy.  Results are identical either way;
+ * which is faster depends on the grid, and fl_autotune times both. */
 int fl_set_stream_hints(int on);
 int fl_get_stream_hints(void);
 
-/* Time every candidate (ty, zt), and for the fused variant both stream-hint settings, for
+/* Time every candidate (ty, zt), with and without stream hints, for
  * `steps` steps on the initialised grid and keep the fastest; fields are restored
  * afterwards.  When cache_path is not NULL the choice is looked up / appended there, keyed
  * by device name, grid, variant and precision. */

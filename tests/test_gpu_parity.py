@@ -212,8 +212,9 @@ def test_tile_shapes_are_bitwise_identical(fl, drv, tiles):
         assert np.array_equal(normal_part(out[0][k]), normal_part(out[1][k]))
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp32r", "bf16", "fp16r", "q16r"])
-def test_stream_hints_are_bitwise_identical(fl, drv, prec):
+@pytest.mark.parametrize("variant,prec", [("fused", "fp32"), ("fused", "fp32r"), ("fused", "bf16"),
+                                          ("fused", "fp16r"), ("fused", "q16r"), ("twopass", "fp32")])
+def test_stream_hints_are_bitwise_identical(fl, drv, variant, prec):
     """The streaming cache hints change only the cache policy of the previous-level and
     coefficient accesses: the fields after 40 steps are the same bits with and without."""
     m, dt = hetero(fl, (60, 33, 41), 6)
@@ -221,7 +222,7 @@ def test_stream_hints_are_bitwise_identical(fl, drv, prec):
     isrc = fl.index(shape, 20, 17, 15)
     out = []
     for h in (False, True):
-        d = drv(variant="fused", precision=prec, tiles=(12, 0))
+        d = drv(variant=variant, precision=prec, tiles=(12, 0))
         d.set_stream_hints(h)
         assert d.stream_hints() == h
         d.initialize(shape, D, dt, m)

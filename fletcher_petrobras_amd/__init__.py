@@ -76,6 +76,7 @@ def lib():
         "fl_record_traces": (i, [i, i, i]),
         "fl_get_traces": (i, [_f32p, i]),
         "fl_run_steps": (i, [i, i, i, _f32p, i]),
+        "fl_reset_fields": (i, []),
         "fl_synchronize": (i, []),
         "fl_set_timing": (i, [i]),
         "fl_kernel_times": (i, [C.POINTER(d), C.POINTER(i)]),
@@ -93,6 +94,7 @@ def lib():
         "fl_batch_get_traces": (i, [i, _f32p, i]),
         "fl_batch_autotune": (i, [i, C.c_char_p]),
         "fl_batch_kernel_ms": (i, [C.POINTER(d)]),
+        "fl_batch_reset": (i, []),
         "fl_batch_destroy": (None, []),
         "fl_ref64_create": (i, [i, i, i, i, f, f, f, f] + [_f32p] * 6),
         "fl_ref64_run": (i, [i, i, i, _f32p, i, i, i]),
@@ -125,6 +127,9 @@ def lib():
         "fl_extract_window": (None, [_f32p] + [i] * 9 + [_f32p]),
         "fl_rsf_write_field": (i, [C.c_char_p] + [i] * 9 + [f, f, f, _f32p]),
         "fl_rsf_write_traces": (i, [C.c_char_p, i, i, i, f, f, f, _f32p]),
+        "fl_rsf_open": (vp, [C.c_char_p] + [i] * 6 + [f] * 4),
+        "fl_rsf_append": (i, [vp, i, i, i, _f32p]),
+        "fl_rsf_close": (i, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -347,6 +352,10 @@ class Driver:
             src = sources(self.dt, st, it0)
         _check(self.L.fl_run_steps(it0, st, isource, _c32(src), graph_chunk))
 
+    def reset_fields(self):
+        """All fields and recorded traces back to zero, device state and graphs kept."""
+        _check(self.L.fl_reset_fields())
+
     def step_loop(self, st, isource, it0=0):
         """The same steps issued call by call through DRIVER_InsertSource/DRIVER_Propagate."""
         for it in range(it0 + 1, it0 + st + 1):
@@ -449,6 +458,10 @@ class Batch:
         _check(self.L.fl_batch_run(it0, st, _c32(src), graph_chunk, rec_every, max_traces))
         if rec_every:
             self.rec = (rec_every, max_traces)
+
+    def reset(self):
+        """Every shot's fields and traces back to zero."""
+        _check(self.L.fl_batch_reset())
 
     def field(self, shot, which):
         out = np.empty(self.shapes[shot], np.float32)

@@ -10,6 +10,7 @@ produces
 Host C is compiled by gcc (-fopenmp for the model/precompute loops); device code by
 hipcc for gfx950 with packed-FP32 instructions disabled.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -24,7 +25,11 @@ ARCH = os.environ.get("FL_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HOST_SRC = ["fl_model.c", "fl_synth.c", "fl_run.c"]
-HIP_SRC = ["fl_driver.hip"]
+# (source, object, extra flags): the driver, and the step kernels of each tile height for the
+# single grid and for batched shots in translation units of their own, compiled in parallel
+HIP_UNITS = [("fl_driver.hip", "fl_driver.o", [])] + [
+    ("fl_inst.hip", "fl_inst_%s%d.o" % (part, ty), ["-DFL_TY=%d" % ty, "-DFL_PART_%s" % part.upper()])
+    for part in ("grid", "batch") for ty in (4, 8, 12, 16)]
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
     # FMAs are written explicitly in the step kernels: no implicit contraction, so every
@@ -55,11 +60,15 @@ def build(verbose=False):
         _run(["gcc", "-O2", "-fPIC", "-fopenmp", "-std=gnu11", "-Wall", "-Wno-unknown-pragmas",
               "-I" + INC, "-c", os.path.join(PKG, "host", src), "-o", o], quiet=not verbose)
         objs.append(o)
-    for src in HIP_SRC:
-        o = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC] + HIP_FLAGS + ["-I" + INC, "-c", os.path.join(PKG, "csrc", src), "-o", o],
-             quiet=not verbose)
+    jobs = []
+    for src, obj, extra in HIP_UNITS:
+        o = os.path.join(BUILD, obj)
+        jobs.append([HIPCC] + HIP_FLAGS + extra + ["-I" + INC, "-c", os.path.join(PKG, "csrc", src), "-o", o])
         objs.append(o)
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as ex:
+        for f in [ex.submit(_run, j, not verbose) for j in jobs]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lgomp"],
          quiet=not verbose)

@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "fl_step.h"
+#include "fl_launch.h"
 #include "fletcher/driver.h"
 #include "fletcher/fl_hip.h"
 #include "fletcher/fl_model.h"
@@ -75,8 +75,8 @@ struct Device {
     float *dvx = nullptr, *dvy = nullptr;  // two-pass (pDx, qDx), (pDy, qDy) pairs
     float *trace = nullptr;
     int rec_iz = -1, rec_every = 0, rec_slots = 0;
-    int *ctr = nullptr;     // step counters for captured runs (2 ints)
-    float *src_tab = nullptr;
+    int *run = nullptr;     // control block of multi-step runs (RUN_WORDS ints)
+    float *src_tab = nullptr;  // source samples of the current multi-step run
     int src_cap = 0;
     long long bytes = 0;
 };
@@ -205,6 +205,18 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
             else
                 fl_pack_coefs16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
                                 cm == CM_BF16, c);
+            if (cm == CM_F16 || cm == CM_F16R) {
+                // FP16 velocity factors v^2 dt^2 (coefficients 6-9, or recompute inputs 0-3)
+                // overflow to infinity at 65520: v dt above about 256 m
+                const int k0 = cm == CM_F16 ? 6 : 0;
+                for (int k = k0; k < k0 + 4; k++)
+                    for (long long i = 0; i < m; i++)
+                        if ((c[k][i] & 0x7c00u) == 0x7c00u)
+                            return fail(FL_ERR_ARG,
+                                        "velocity factor v^2 dt^2 of cell %lld exceeds the FP16 range "
+                                        "(v dt > 256 m): use FP32, BF16 or block floating-point storage",
+                                        (long long)off + i);
+            }
             const uint16_t *co[10];
             for (int k = 0; k < nk; k++)
                 co[k] = c[(cm == CM_F16R || cm == CM_Q16R) ? rec_order_r16(k) : rec_order16(k)];
@@ -265,52 +277,13 @@ static void fill_geom(StepArgs &a, int sx, int sy, int sz, int bord, int ty, int
     a.nblk = a.ntx * a.nty * nzc;
 }
 
-template <int TY, int CM, bool TP, bool ISO, bool HINTS = false>
-static void launch_zm(const StepArgs &a, hipStream_t s)
-{
-    const int g = (a.nblk + 7) / 8 * 8;
-    hipLaunchKernelGGL((tti_step<TY, CM, TP, ISO, HINTS>), dim3(g), dim3(64, TY), 0, s, a);
-}
-
 // Equal spacings select the kernels that scale the right-hand side once (ISO).
 static bool iso_spacing(float dx, float dy, float dz) { return dx == dy && dy == dz; }
 
 static int launch_steps(const StepArgs &a, int ty, int cm, bool tp, hipStream_t s)
 {
-    const bool iso = iso_spacing(D.dx, D.dy, D.dz);
-    const bool hints = g_tiles.hints != 0;
-#define FL_ZM1(TY, CM)                                                     \
-    case CM:                                                               \
-        if (iso) {                                                         \
-            if (hints) launch_zm<TY, CM, false, true, true>(a, s);         \
-            else launch_zm<TY, CM, false, true>(a, s);                     \
-        } else {                                                           \
-            if (hints) launch_zm<TY, CM, false, false, true>(a, s);        \
-            else launch_zm<TY, CM, false, false>(a, s);                    \
-        }                                                                  \
+    if (launch_grid(ty, a, cm, tp, iso_spacing(D.dx, D.dy, D.dz), g_tiles.hints != 0, s) == 0)
         return FL_OK;
-#define FL_ZM(TY)                                                          \
-    if (ty == TY) {                                                        \
-        if (tp) {                                                          \
-            if (hints) launch_zm<TY, CM_F32, true, false, true>(a, s);     \
-            else launch_zm<TY, CM_F32, true, false>(a, s);                 \
-            return FL_OK;                                                  \
-        }                                                                  \
-        switch (cm) {                                                      \
-            FL_ZM1(TY, CM_F32)                                             \
-            FL_ZM1(TY, CM_BF16)                                            \
-            FL_ZM1(TY, CM_F16)                                             \
-            FL_ZM1(TY, CM_F16R)                                            \
-            FL_ZM1(TY, CM_F32R)                                            \
-            FL_ZM1(TY, CM_Q16R)                                            \
-        }                                                                  \
-    }
-    FL_ZM(4)
-    FL_ZM(8)
-    FL_ZM(12)
-    FL_ZM(16)
-#undef FL_ZM
-#undef FL_ZM1
     return fail(FL_ERR_ARG, "unsupported tile rows %d / coefficient mode %d", ty, cm);
 }
 
@@ -356,28 +329,50 @@ static int ensure_dvol()
     return FL_OK;
 }
 
-// One field (fld 0 p, 1 q) of a level from / to a reference-layout host volume, through a
-// linear device buffer.
+// Staging buffer of host transfers: z slabs of one field, at most 64 MiB, kept between
+// transfers (released by DRIVER_Finalize and fl_batch_destroy).
+static float *g_xfer = nullptr;
+static size_t g_xfer_bytes = 0;
+
+static void free_xfer()
+{
+    if (g_xfer) (void)hipFree(g_xfer);
+    g_xfer = nullptr;
+    g_xfer_bytes = 0;
+}
+
+// One field (fld 0 p, 1 q) of a level from / to a reference-layout host volume, slab by
+// slab through the staging buffer.
 static int level_xfer(float *lv, int fld, float *host, bool put, int sx, int sy, int sz, int px)
 {
-    const long long n = (long long)sx * sy * sz;
-    float *tmp = nullptr;
-    long long dummy = 0;
-    int rc = dev_alloc((void **)&tmp, (size_t)n * 4, &dummy);
-    if (rc) return rc;
-    hipError_t e = hipStreamSynchronize(g_stream);
-    const dim3 g((sx + 255) / 256, sy, sz);
-    if (put) {
-        if (e == hipSuccess) e = hipMemcpy(tmp, host, (size_t)n * 4, hipMemcpyHostToDevice);
-        if (e == hipSuccess) hipLaunchKernelGGL(field_pack, g, dim3(256), 0, g_stream, tmp, lv, fld, sx, sy, px);
-        if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
-    } else {
-        hipLaunchKernelGGL(field_unpack, g, dim3(256), 0, g_stream, lv, tmp, fld, sx, sy, px);
-        if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
-        if (e == hipSuccess) e = hipMemcpy(host, tmp, (size_t)n * 4, hipMemcpyDeviceToHost);
+    const long long plane = (long long)sx * sy;
+    const int zs = (int)std::max<long long>(1, std::min<long long>(sz, (64ll << 20) / (plane * 4)));
+    const size_t need = (size_t)zs * plane * 4;
+    if (g_xfer_bytes < need) {
+        free_xfer();
+        long long dummy = 0;
+        int rc = dev_alloc((void **)&g_xfer, need, &dummy);
+        if (rc) return rc;
+        g_xfer_bytes = need;
     }
-    if (e == hipSuccess) e = hipGetLastError();
-    (void)hipFree(tmp);
+    hipError_t e = hipStreamSynchronize(g_stream);
+    for (int z0 = 0; z0 < sz && e == hipSuccess; z0 += zs) {
+        const int nz = std::min(zs, sz - z0);
+        float *slab = lv + 2ll * z0 * sy * px;
+        float *h = host + z0 * plane;
+        const size_t bytes = (size_t)nz * plane * 4;
+        const dim3 g((sx + 255) / 256, sy, nz);
+        if (put) {
+            e = hipMemcpy(g_xfer, h, bytes, hipMemcpyHostToDevice);
+            if (e == hipSuccess) hipLaunchKernelGGL(field_pack, g, dim3(256), 0, g_stream, g_xfer, slab, fld, sx, sy, px);
+            if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+        } else {
+            hipLaunchKernelGGL(field_unpack, g, dim3(256), 0, g_stream, slab, g_xfer, fld, sx, sy, px);
+            e = hipStreamSynchronize(g_stream);
+            if (e == hipSuccess) e = hipMemcpy(h, g_xfer, bytes, hipMemcpyDeviceToHost);
+        }
+        if (e == hipSuccess) e = hipGetLastError();
+    }
     if (e != hipSuccess) return fail(FL_ERR_HIP, "field transfer: %s", hipGetErrorString(e));
     return FL_OK;
 }
@@ -442,6 +437,40 @@ static int take_events(hipEvent_t *e0, hipEvent_t *e1)
     return FL_OK;
 }
 
+// ------------------------------------------------------------------ graphs
+// Instantiated graph of K captured steps.  It is replayed while the launch arguments of
+// its first two steps (levels, tiles, coefficient records, traces, source cell, control
+// block) and the kernel selection are those it was captured with; two are kept, one per
+// level parity at the start of a run.
+struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int K = 0, ty = 0, cm = 0, hints = 0, variant = 0;
+    StepArgs a0, a1;
+    unsigned long long used = 0;
+};
+static GraphSlot g_graphs[2];
+static unsigned long long g_graph_tick = 0;
+
+static void drop_graphs()
+{
+    for (auto &g : g_graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        g = GraphSlot();
+    }
+}
+
+__global__ void set_run(int *run, int it_base, int src_n)
+{
+    if (threadIdx.x == 0) {
+        run[RUN_K0] = 0;
+        run[RUN_K1] = 0;
+        run[RUN_IT_BASE] = it_base;
+        run[RUN_SRC_N] = src_n;
+    }
+}
+
 // ------------------------------------------------------------------ driver.h
 static void free_device()
 {
@@ -452,8 +481,10 @@ static void free_device()
     if (D.dvx) (void)hipFree(D.dvx);
     if (D.dvy) (void)hipFree(D.dvy);
     if (D.trace) (void)hipFree(D.trace);
-    if (D.ctr) (void)hipFree(D.ctr);
+    if (D.run) (void)hipFree(D.run);
     if (D.src_tab) (void)hipFree(D.src_tab);
+    drop_graphs();
+    free_xfer();
     D = Device();
 }
 
@@ -508,7 +539,7 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
         free_device();
         return rc;
     }
-    if ((rc = dev_alloc((void **)&D.ctr, 2 * sizeof(int), &D.bytes))) { free_device(); return rc; }
+    if ((rc = dev_alloc((void **)&D.run, RUN_WORDS * sizeof(int), &D.bytes))) { free_device(); return rc; }
     D.init = true;
     return FL_OK;
 }
@@ -724,7 +755,60 @@ extern "C" int fl_get_traces(float *host, int max_traces)
     return m;
 }
 
-__global__ void set_int(int *p, int v) { if (threadIdx.x == 0) *p = v; }
+// The graph of K steps starting from the current levels, captured and instantiated on
+// first use.  args(k) gives the launch arguments of step k.
+template <class F>
+static int step_graph(int K, F &&args, GraphSlot **out)
+{
+    const StepArgs a0 = args(0), a1 = args(1);
+    for (auto &g : g_graphs)
+        if (g.exec && g.K == K && g.ty == g_tiles.ty && g.cm == D.cm && g.hints == g_tiles.hints &&
+            g.variant == g_variant && memcmp(&g.a0, &a0, sizeof(StepArgs)) == 0 &&
+            memcmp(&g.a1, &a1, sizeof(StepArgs)) == 0) {
+            g.used = ++g_graph_tick;
+            *out = &g;
+            return FL_OK;
+        }
+    GraphSlot *g = &g_graphs[0];
+    for (auto &c : g_graphs)
+        if (c.used < g->used) g = &c;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    *g = GraphSlot();
+    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    const int lc0 = D.lc;
+    int rc = FL_OK;
+    for (int k = 0; k < K && rc == FL_OK; k++) {
+        rc = launch_steps(args(k), g_tiles.ty, D.cm, false, g_stream);
+        swap_levels();
+    }
+    D.lc = lc0;  // K is even: the levels are back in their roles after a replay
+    hipGraph_t graph = nullptr;
+    hipError_t ec = hipStreamEndCapture(g_stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(FL_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+    hipGraphExec_t exec = nullptr;
+    ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ec != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(FL_ERR_HIP, "graph instantiation: %s", hipGetErrorString(ec));
+    }
+    g->exec = exec;
+    g->graph = graph;
+    g->K = K;
+    g->ty = g_tiles.ty;
+    g->cm = D.cm;
+    g->hints = g_tiles.hints;
+    g->variant = g_variant;
+    g->a0 = a0;
+    g->a1 = a1;
+    g->used = ++g_graph_tick;
+    *out = g;
+    return FL_OK;
+}
 
 extern "C" int fl_run_steps(int it0, int nsteps, int iSource, const float *src, int graph_chunk)
 {
@@ -752,48 +836,43 @@ extern "C" int fl_run_steps(int it0, int nsteps, int iSource, const float *src, 
         if (g_sync) HIPCHK(hipStreamSynchronize(g_stream));
         return FL_OK;
     }
+    // source samples: a device table grown in powers of two, so that graphs stay valid
+    // across runs of up to its capacity
     if (D.src_cap < nsteps) {
-        if (D.src_tab) (void)hipFree(D.src_tab);
+        int cap = 4096;
+        while (cap < nsteps) cap *= 2;
+        if (D.src_tab) {
+            HIPCHK(hipStreamSynchronize(g_stream));
+            (void)hipFree(D.src_tab);
+            D.bytes -= (long long)D.src_cap * 4;
+        }
         D.src_tab = nullptr;
-        int rc = dev_alloc((void **)&D.src_tab, (size_t)nsteps * 4, &D.bytes);
+        D.src_cap = 0;
+        int rc = dev_alloc((void **)&D.src_tab, (size_t)cap * 4, &D.bytes);
         if (rc) return rc;
-        D.src_cap = nsteps;
+        D.src_cap = cap;
     }
     HIPCHK(hipMemcpyAsync(D.src_tab, src, (size_t)nsteps * 4, hipMemcpyHostToDevice, g_stream));
-    hipLaunchKernelGGL(set_int, dim3(1), dim3(64), 0, g_stream, D.ctr, 0);
+    hipLaunchKernelGGL(set_run, dim3(1), dim3(64), 0, g_stream, D.run, it0, nsteps);
     const long long psrc = pitched_index(iSource, D.sx, D.sy, D.px);
     hipLaunchKernelGGL(insert_source, dim3(1), dim3(64), 0, g_stream, D.lv[D.lc], psrc, src[0]);
-    const int K = std::min(nsteps, (graph_chunk + 1) / 2 * 2);  // even: levels return to their roles
+    const int K = std::min(nsteps, graph_chunk) & ~1;  // even: levels return to their roles
     auto step_args = [&](int k) {
         StepArgs a = base_args();
-        a.it_base = it0;
-        a.it_dev = D.ctr + (k & 1);
-        a.it_next = D.ctr + ((k + 1) & 1);
+        a.run = D.run;
+        a.par = k & 1;
         a.src_tab = D.src_tab;
-        a.src_n = nsteps;
         a.src_idx = psrc;
         return a;
     };
     int done = 0;
     if (K >= 2 && nsteps >= K) {
-        hipGraph_t graph;
-        hipGraphExec_t exec;
-        HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
-        int rc = FL_OK;
-        for (int k = 0; k < K && rc == FL_OK; k++) {
-            rc = launch_steps(step_args(k), g_tiles.ty, D.cm, false, g_stream);
-            swap_levels();
-        }
-        hipError_t ec = hipStreamEndCapture(g_stream, &graph);
+        GraphSlot *g = nullptr;
+        int rc = step_graph(K, step_args, &g);
         if (rc) return rc;
-        if (ec != hipSuccess) return fail(FL_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
-        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         const int reps = nsteps / K;
-        for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(exec, g_stream));
+        for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(g->exec, g_stream));
         done = reps * K;
-        HIPCHK(hipStreamSynchronize(g_stream));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
     }
     for (int k = done; k < nsteps; k++) {
         int rc = launch_steps(step_args(k), g_tiles.ty, D.cm, false, g_stream);
@@ -801,6 +880,18 @@ extern "C" int fl_run_steps(int it0, int nsteps, int iSource, const float *src, 
         swap_levels();
     }
     HIPCHK(hipGetLastError());
+    if (g_sync) HIPCHK(hipStreamSynchronize(g_stream));
+    return FL_OK;
+}
+
+extern "C" int fl_reset_fields(void)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "fl_reset_fields before DRIVER_Initialize");
+    for (int k = 0; k < 2; k++) HIPCHK(hipMemsetAsync(D.lv[k], 0, (size_t)2 * D.np * 4, g_stream));
+    if (D.trace)
+        HIPCHK(hipMemsetAsync(D.trace, 0, (size_t)D.rec_slots * (D.sx - 2 * D.bord) * (D.sy - 2 * D.bord) * 4,
+                              g_stream));
+    D.lc = 1;
     if (g_sync) HIPCHK(hipStreamSynchronize(g_stream));
     return FL_OK;
 }

@@ -103,15 +103,16 @@ struct StepArgs {
     int sx, sy, sz, bord, px;
     int ntx, nty, zt, nblk;  // tiles along x, y; z chunk depth; logical blocks
     int xorg;                // x of the first tile column (0 or bord)
-    // step index it: it_host, or it_base + 1 + *it_dev inside a captured run, where
-    // the kernel also publishes *it_next = *it_dev + 1 for the following step
-    int it_host, it_base;
-    const int *it_dev;
-    int *it_next;
-    // source of the next step added to this step's output (captured runs); src_idx is
-    // the pitched cell index
+    // step index it: it_host for a plain launch.  Inside a multi-step run, run points at
+    // the run's control block {k0, k1, it_base, src_n}: the step reads its counter
+    // k = run[par], takes it = it_base + 1 + k and publishes run[1 - par] = k + 1 for the
+    // following step, so a captured graph replays for any start step and run length.
+    int it_host;
+    int *run;
+    int par;
+    // source of the next step added to this step's output (multi-step runs): src_tab[k+1]
+    // while k + 1 < src_n; src_idx is the pitched cell index
     const float *src_tab;
-    int src_n;
     long long src_idx;
     // receiver plane: slot it/rec_every-1 of trace when it % rec_every == 0
     float *trace;
@@ -125,15 +126,19 @@ struct StepCtl {
     bool do_src;
 };
 
+// Control block of a multi-step run (StepArgs.run, BatchArgs.run).
+enum { RUN_K0 = 0, RUN_K1 = 1, RUN_IT_BASE = 2, RUN_SRC_N = 3, RUN_WORDS = 4 };
+
 __device__ __forceinline__ StepCtl step_ctl(const StepArgs &a)
 {
     StepCtl c;
-    int k = 0, it = a.it_host;
-    if (a.it_dev) {
-        k = *a.it_dev;
-        it = a.it_base + 1 + k;
+    int k = 0, it = a.it_host, src_n = 0;
+    if (a.run) {
+        k = a.run[a.par];
+        it = a.run[RUN_IT_BASE] + 1 + k;
+        src_n = a.run[RUN_SRC_N];
     }
-    c.do_src = a.src_tab && (k + 1) < a.src_n;
+    c.do_src = a.src_tab && (k + 1) < src_n;
     c.src_add = c.do_src ? a.src_tab[k + 1] : 0.0f;
     c.trace = nullptr;
     if (a.trace && a.rec_every > 0 && it % a.rec_every == 0) {
@@ -243,6 +248,9 @@ __device__ __forceinline__ int xcd_remap(int b, int g)
 }
 
 // ---------------------------------------------------------------------------------
+// Kernels launched by the driver translation unit only (the kernel instantiation units
+// define FL_NO_AUX_KERNELS).
+#ifndef FL_NO_AUX_KERNELS
 // Direct 64-point DerCross form (original/sample.h:32-90), global loads through L1/L2.
 __device__ __forceinline__ float cross_group(const float *__restrict__ p, long long i, long long s11,
                                              long long s21, int m, int n)
@@ -380,5 +388,7 @@ __global__ __launch_bounds__(256) void field_unpack(const float *__restrict__ lv
     const long long row = (long long)blockIdx.z * sy + blockIdx.y;
     lin[row * sx + x] = lv[2 * (row * px + x) + fld];
 }
+
+#endif  // FL_NO_AUX_KERNELS
 
 }  // namespace fl

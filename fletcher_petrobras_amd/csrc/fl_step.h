@@ -597,9 +597,9 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(step_wa
 void tti_step(const StepArgs a)
 {
     __shared__ StepLds<TY> s;
-    if (a.it_next && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0)
-        *a.it_next = (a.it_dev ? *a.it_dev : 0) + 1;
     const StepCtl ctl = step_ctl(a);
+    if (a.run && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0)
+        a.run[1 - a.par] = a.run[a.par] + 1;
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     if (t >= a.nblk) return;
     const int tix = t % a.ntx;
@@ -621,16 +621,17 @@ struct ShotDev {
 
 struct BatchArgs {
     const ShotDev *shots;
+    const int *item_shot;  // shot of each work item
     int nshots, nitems, parity, zt;
     float dxxinv, dyyinv, dzzinv, dxinv, dyinv, dzinv, dt;
     int bord;
-    int it_base;
-    const int *it_dev;
-    int *it_next;
+    int *run;  // run control block {k0, k1, it_base, src_n}; this step reads run[par]
+    int par;
     const float *src_tab;
-    int src_n, rec_every, rec_slots;
+    int rec_every, rec_slots;
 };
 
+#ifndef FL_NO_AUX_KERNELS
 // Source of the first step of a batched run: every shot's pc, qc at its source cell.
 __global__ void batch_insert(const ShotDev *shots, int nshots, int parity, float s)
 {
@@ -640,6 +641,7 @@ __global__ void batch_insert(const ShotDev *shots, int nshots, int parity, float
     sh.lv[1 - parity][2 * sh.src_idx] += s;
     sh.lv[1 - parity][2 * sh.src_idx + 1] += s;
 }
+#endif
 
 // Batched shots: one workgroup per (shot, tile) item of the work list.
 // The batched kernel keeps a shot descriptor besides the tile state (about 10 VGPRs more
@@ -661,17 +663,13 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(batch_w
 void tti_step_batch(const BatchArgs b)
 {
     __shared__ StepLds<TY> s;
-    const int k = *b.it_dev;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *b.it_next = k + 1;
-    const int it = b.it_base + 1 + k;
+    const int k = b.run[b.par];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) b.run[1 - b.par] = k + 1;
+    const int it = b.run[RUN_IT_BASE] + 1 + k;
+    const int src_n = b.run[RUN_SRC_N];
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     if (t >= b.nitems) return;
-    int lo = 0, hi = b.nshots - 1;  // shot owning item t
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (b.shots[mid].item0 <= t) lo = mid; else hi = mid - 1;
-    }
-    const ShotDev &sh = b.shots[__builtin_amdgcn_readfirstlane(lo)];
+    const ShotDev &sh = b.shots[__builtin_amdgcn_readfirstlane(b.item_shot[t])];
     StepArgs a;
     a.prev = sh.lv[b.parity];
     a.cur = sh.lv[1 - b.parity];
@@ -685,7 +683,7 @@ void tti_step_batch(const BatchArgs b)
     a.rec_iz = sh.rec_iz;
     a.src_idx = sh.src_idx;
     StepCtl ctl;
-    ctl.do_src = b.src_tab && (k + 1) < b.src_n;
+    ctl.do_src = b.src_tab && (k + 1) < src_n;
     ctl.src_add = ctl.do_src ? b.src_tab[k + 1] : 0.0f;
     ctl.trace = nullptr;
     if (sh.trace && b.rec_every > 0 && it % b.rec_every == 0) {

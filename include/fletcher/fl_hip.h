@@ -92,8 +92,14 @@ int fl_get_traces(float *host, int max_traces);
 
 /* nsteps steps it0+1 .. it0+nsteps of the reference loop (model.c:71-88): before step
  * it0+1+k, src[k] is added to pc and qc at iSource.  Steps are captured in HIP graphs
- * of graph_chunk steps (0: no graph) and replayed. */
+ * of graph_chunk steps (0: no graph) and replayed.  The instantiated graph is kept and
+ * replayed by later calls with the same chunk, tiles, traces and source cell (any it0 and
+ * nsteps): only the first such call pays for capture and instantiation. */
 int fl_run_steps(int it0, int nsteps, int iSource, const float *src, int graph_chunk);
+
+/* All four fields (and the recorded traces) back to zero, levels in their initial roles,
+ * device state and graphs kept: a run can be repeated from rest. */
+int fl_reset_fields(void);
 
 int fl_synchronize(void);
 
@@ -134,7 +140,11 @@ int fl_batch_run(int it0, int nsteps, const float *src, int graph_chunk, int rec
 int fl_batch_get_field(int shot, int which, float *host);
 int fl_batch_get_traces(int shot, float *host, int max_traces);
 int fl_batch_autotune(int steps, const char *cache_path);
-int fl_batch_kernel_ms(double *ms); /* device time of the last fl_batch_run */
+/* Device time of the last fl_batch_run: the source upload, first insertion and the steps,
+ * measured after the work list and the graph are in place. */
+int fl_batch_kernel_ms(double *ms);
+/* Every shot's fields and traces back to zero (as fl_reset_fields). */
+int fl_batch_reset(void);
 void fl_batch_destroy(void);
 
 /* ---------------- FP64 accuracy reference on the device ----------------

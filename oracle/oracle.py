@@ -17,7 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
-VARIANTS = ("original", "der1der1", "der1der1lm")
+VARIANTS = ("original", "der1der1", "der1der1hm", "der1der1lm")
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -98,7 +98,9 @@ def run(variant, shape, bord, d, dt, st, isource, coefs, fields=None, fp="f32",
         rec_iz=-1, rec_every=0, src=None):
     """Run st steps of the reference time loop on the oracle.
 
-    variant: "original" (direct DerCross) or "der1der1" (two-pass first-of-first).
+    variant: "original" (direct DerCross), "der1der1" (two-pass first-of-first), or its
+    loop orders "der1der1hm" (first derivatives formed plane by plane in z order) and
+    "der1der1lm" (first derivatives in a 9-plane ring).
     shape: (sz, sy, sx); d: (dx, dy, dz); coefs: 10 arrays of dtype fp.
     Returns dict with pp, pc, qp, qc (pc/qc = field at t=st*dt) and traces.
     """
@@ -119,7 +121,7 @@ def run(variant, shape, bord, d, dt, st, isource, coefs, fields=None, fp="f32",
         traces = np.zeros((nt, sy - 2 * bord, sx - 2 * bord), dtype)
         tptr = traces.ctypes.data
     f = getattr(lib(), "oracle_run_" + fp)
-    v = {"original": 0, "der1der1": 1}[variant]
+    v = {"original": 0, "der1der1": 1, "der1der1hm": 2, "der1der1lm": 3}[variant]
     ntr = f(v, sx, sy, sz, bord, d[0], d[1], d[2], dt, st, isource, src, _ptrs(coefs),
             fields[0], fields[1], fields[2], fields[3], rec_iz, rec_every, tptr)
     if ntr < 0:

@@ -314,3 +314,126 @@ def test_pack_recompute16_inputs(fl):
         back = out[k].astype(np.float64) / 65536.0 * 2 * np.pi
         d = np.abs(np.angle(np.exp(1j * (back - ang.astype(np.float64)))))
         assert d.max() <= np.pi / 65536 * (1 + 1e-9)
+
+
+REF_D1 = os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "libref_der1der1.so")
+have_ref_d1 = pytest.mark.skipif(not os.path.exists(REF_D1), reason="reference build oracle/_ref absent")
+
+
+def _dump_steps(st, dt, dt_output=0.01):
+    """Steps after which model.c:97-102 dumps pc: tSim = it*dt >= tOut = nOut*dtOutput, in
+    float arithmetic as the reference evaluates it."""
+    out, n_out = [], 1
+    t_out = np.float32(n_out) * np.float32(dt_output)
+    for it in range(1, st + 1):
+        if np.float32(it) * np.float32(dt) >= t_out:
+            out.append(it)
+            n_out += 1
+            t_out = np.float32(n_out) * np.float32(dt_output)
+    return out
+
+
+@have_ref_d1
+def test_rsf_stream_matches_reference_der1der1_dump(fl, oracle, tmp_path):
+    """The reference der1der1 front-end writes TTI.rsf / TTI.rsf@: the initial pc, then pc
+    every 0.01 s of simulated time (der1der1/main.c:273-300, model.c:97-102, utils.c:71-205).
+    fl_rsf_open / fl_rsf_append / fl_rsf_close fed the same fields (the oracle's, bitwise
+    equal to the reference's) write the same header text and the same binary bytes."""
+    ref_dir = tmp_path / "ref"
+    ref_dir.mkdir()
+    cap = oracle.Ref("der1der1").main(["TTI", 8, 8, 8, 4, 12.5, 12.5, 12.5, 0.001, 0.025],
+                                      workdir=str(ref_dir))
+    shape, st, isrc, dt = cap["shape"], cap["st"], cap["isource"], cap["dt"]
+    steps = _dump_steps(st, dt)
+    assert steps == [10, 20]
+    model = [cap[k] for k in ("vpz", "vsv", "epsilon", "delta", "phi", "theta")]
+    coefs = oracle.precompute(model)
+    frames = [np.zeros(shape, np.float32)]
+    frames += [oracle.run("der1der1", shape, 4, (12.5, 12.5, 12.5), dt, s, isrc, coefs)["pc"] for s in steps]
+    assert np.array_equal(frames[-1], oracle.run("der1der1", shape, 4, (12.5,) * 3, dt, 20, isrc, coefs)["pc"])
+    mine = tmp_path / "mine"
+    mine.mkdir()
+    sz, sy, sx = shape
+    cwd = os.getcwd()
+    os.chdir(mine)
+    try:
+        L = fl.lib()
+        h = L.fl_rsf_open(b"TTI", 0, sx - 1, 0, sy - 1, 0, sz - 1, 12.5, 12.5, 12.5, dt)
+        assert h
+        for fr in frames:
+            assert L.fl_rsf_append(h, sx, sy, sz, np.ascontiguousarray(fr)) == 0
+        assert L.fl_rsf_close(h) == 0
+    finally:
+        os.chdir(cwd)
+    for name in ("TTI.rsf", "TTI.rsf@"):
+        a = (ref_dir / name).read_bytes()
+        b = (mine / name).read_bytes()
+        assert a == b, name
+    assert (ref_dir / "TTI.rsf").read_text().splitlines()[0] == 'in="./TTI.rsf@"'
+
+
+@have_ref_d1
+def test_rsf_field_dump_matches_reference(fl, oracle, tmp_path):
+    """fl_rsf_write_field writes a window as the reference's DumpFieldToFile
+    (utils.c:7-48) does: same header bytes, same binary bytes."""
+    R = oracle.Ref("der1der1")
+    R.L.DumpFieldToFile.restype = None
+    R.L.DumpFieldToFile.argtypes = [C.c_int] * 9 + [C.c_float] * 3 + [C.c_void_p, C.c_char_p]
+    a = np.random.default_rng(3).standard_normal((7, 6, 9)).astype(np.float32)
+    win = (1, 7, 0, 4, 2, 6)
+    for sub, call in (("ref", lambda: R.L.DumpFieldToFile(9, 6, 7, *win, 12.5, 10.0, 7.5, a.ctypes.data, b"snap")),
+                      ("mine", lambda: fl.lib().fl_rsf_write_field(b"snap", 9, 6, 7, *win, 12.5, 10.0, 7.5, a))):
+        d = tmp_path / sub
+        d.mkdir()
+        cwd = os.getcwd()
+        os.chdir(d)
+        try:
+            call()
+        finally:
+            os.chdir(cwd)
+    for name in ("snap.rsf", "snap.rsf@"):
+        assert (tmp_path / "ref" / name).read_bytes() == (tmp_path / "mine" / name).read_bytes(), name
+    data = np.fromfile(str(tmp_path / "mine" / "snap.rsf@"), np.float32).reshape(5, 5, 7)
+    assert np.array_equal(data, a[2:7, 0:5, 1:8])
+
+
+def test_trace_rsf_writer(fl, tmp_path):
+    """Receiver traces as a 3-D RSF cube: n1 = x, n2 = y, n3 = recorded steps, in the header
+    layout of utils.c:28-37 (in=, data_format, esize, n/d per axis)."""
+    tr = np.random.default_rng(5).standard_normal((6, 4, 5)).astype(np.float32)
+    name = str(tmp_path / "traces")
+    assert fl.lib().fl_rsf_write_traces(name.encode(), 5, 4, 6, 12.5, 10.0, 0.004, tr) == 0
+    head = (tmp_path / "traces.rsf").read_text().splitlines()
+    assert head == ['in="./traces.rsf@"', 'data_format="native_float"', "esize=4", "n1=5", "d1=12.500000",
+                    "n2=4", "d2=10.000000", "n3=6", "d3=0.004000"]
+    assert np.array_equal(np.fromfile(name + ".rsf@", np.float32).reshape(6, 4, 5), tr)
+
+
+@have_ref_d1
+def test_literal_config1_model_matches_reference_backend(fl, oracle):
+    """The 88^3 configuration with the stated model vsv = 0, phi = 0 (the reference's TTI
+    branch uses phi = 1 and vsv from epsilon, delta: main.c:202-220): the cross coefficients
+    ch1dxy = sin^2(theta) sin(2 phi) and ch1dyz = sin(2 theta) sin(phi) and v2sz = vsv^2 are
+    exactly 0 inside the model, and the oracle equals the reference back-end driven on the
+    same arrays bit for bit (reduced here to 24^3)."""
+    nx, absorb = 24, 16
+    shape = fl.grid_shape(nx, nx, nx, absorb)
+    m = fl.model_homogeneous("TTI", shape)
+    m[1][:] = 0.0  # vsv
+    m[4][:] = 0.0  # phi
+    fl.random_boundary(m, (nx, nx, nx), absorb)
+    c = fl.precompute(m)
+    inner = (slice(4 + absorb, -4 - absorb),) * 3
+    for k in (3, 4):
+        assert not np.any(c[k]), k
+    assert not np.any(c[8][inner])
+    assert np.all(c[0][inner] > 0) and np.all(c[5][inner] > 0)
+    for k, ck in enumerate(oracle.precompute(m)):
+        assert np.array_equal(ck, c[k]), k
+    R = oracle.Ref("der1der1")
+    R.setup(shape, 4, (12.5,) * 3, 0.001, m)
+    isrc = fl.center_index(shape)
+    R.steps(0, 20, isrc)
+    r = oracle.run("der1der1", shape, 4, (12.5,) * 3, 0.001, 20, isrc, c)
+    assert np.array_equal(r["pc"], R.fetch("pc")) and np.array_equal(r["qc"], R.fetch("qc"))
+    assert float(np.abs(r["pc"]).max()) > 0

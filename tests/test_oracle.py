@@ -44,6 +44,28 @@ def test_oracle_matches_reference_fixture_absorb16(oracle, fl):
     assert np.array_equal(r["pc"], g["pc"]) and np.array_equal(r["qc"], g["qc"])
 
 
+@pytest.mark.parametrize("variant", ["der1der1hm", "der1der1lm"])
+@pytest.mark.parametrize("fp", ["f32", "f64"])
+def test_loop_orders_equal_der1der1(oracle, fl, variant, fp):
+    """der1der1hm (first derivatives formed plane by plane in z order) and der1der1lm (a
+    9-plane ring) restate der1der1's arithmetic in other loop orders: same bits, FP32 and
+    FP64, on a ragged heterogeneous grid and on the reference TTI model."""
+    m = fl.model_gradient((15, 11, 9), 5, seed=11, nblobs=4, blob_smin=2.0, blob_smax=5.0,
+                          smooth_smin=3.0, smooth_smax=8.0)
+    shape = m[0].shape
+    isrc = fl.index(shape, shape[2] // 2, shape[1] // 2, 4 + 5 + 2)
+    c = oracle.precompute(m, fp)
+    a = oracle.run("der1der1", shape, 4, D, 0.0008, 24, isrc, c, fp=fp)
+    b = oracle.run(variant, shape, 4, D, 0.0008, 24, isrc, c, fp=fp)
+    for k in ("pc", "qc", "pp", "qp"):
+        assert np.array_equal(a[k], b[k]), k
+    assert float(np.abs(a["pc"]).max()) > 0
+    g = golden("ref_TTI_der1der1_n8_a4_s30.npz")
+    shape, m = ref_model(fl, "TTI", 8, 4)
+    r = oracle.run(variant, shape, 4, D, 0.001, 30, int(g["isource"]), oracle.precompute(m))
+    assert np.array_equal(r["pc"], g["pc"]) and np.array_equal(r["qc"], g["qc"])
+
+
 def test_lowmem_reference_equals_der1der1():
     """der1der1lm's 9-plane ring (the z-queue precedent) gives der1der1 bit for bit."""
     a = golden("ref_TTI_der1der1lm_n8_a4_s30.npz")
@@ -54,7 +76,7 @@ def test_lowmem_reference_equals_der1der1():
 @pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref",
                                                      "libref_original.so")),
                     reason="reference build oracle/_ref absent")
-@pytest.mark.parametrize("variant", ["original", "der1der1"])
+@pytest.mark.parametrize("variant", ["original", "der1der1", "der1der1hm", "der1der1lm"])
 def test_oracle_matches_live_reference_ragged(oracle, fl, variant):
     """Non-cubic grid, heterogeneous model: the reference back-end driven on the same arrays."""
     n3, absorb = (13, 9, 11), 5

@@ -26,7 +26,7 @@ CLI_PATH = os.path.join(_HERE, "bin", "fletcher")
 BORD = 4  # stencil halo (main.c:21)
 FORM = {"ISO": 0, "VTI": 1, "TTI": 2}
 VARIANT = {"fused": 0, "direct": 1, "twopass": 2}
-PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp16r": 3, "fp32r": 4, "q16r": 5}
+PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp16r": 3, "fp32r": 4, "q16r": 5, "bf16q": 6}
 FIELD = {"pp": 0, "pc": 1, "qp": 2, "qc": 3}
 COEF_NAMES = ("ch1dxx", "ch1dyy", "ch1dzz", "ch1dxy", "ch1dyz", "ch1dxz",
               "v2px", "v2pz", "v2sz", "v2pn")
@@ -85,6 +85,7 @@ def lib():
         "fl_device_name": (i, [C.c_char_p, i]),
         "fl_get_derivative_volume": (i, [i, _f32p]),
         "fl_device_bytes": (ll, []),
+        "fl_device_memory": (i, [C.POINTER(ll), C.POINTER(ll)]),
         "fl_bytes_per_cell": (i, []),
         "fl_batch_create": (i, [i, vp, i, f, f, f, f, i]),
         "fl_batch_set_tiles": (i, [i, i]),
@@ -122,6 +123,7 @@ def lib():
         "fl_pack_coefs16": (None, [C.c_int64] + [_f32p] * 6 + [d, i, C.POINTER(vp)]),
         "fl_pack_recompute16": (None, [C.c_int64] + [_f32p] * 6 + [d, C.POINTER(vp)]),
         "fl_pack_recompute32": (None, [C.c_int64] + [_f32p] * 6 + [C.POINTER(vp)]),
+        "fl_pack_coefs_b16q": (None, [C.c_int64] + [_f32p] * 6 + [d, i, C.POINTER(vp)]),
         "fl_recompute_q16_base": (i, [C.c_int64] + [_f32p] * 4 + [d]),
         "fl_pack_recompute_q16": (None, [C.c_int64] + [_f32p] * 6 + [d, i, C.POINTER(vp)]),
         "fl_extract_window": (None, [_f32p] + [i] * 9 + [_f32p]),
@@ -259,6 +261,13 @@ def model_layered(n3, absorb, seed=1024, bord=BORD, **overrides):
     m = [np.empty(shape, np.float32) for _ in range(6)]
     _check(lib().fl_model_layered(C.byref(p), sx, sy, sz, n3[0], n3[1], n3[2], bord, absorb, *m))
     return m
+
+
+def device_memory():
+    """(free, total) bytes of the device."""
+    f, t = C.c_longlong(), C.c_longlong()
+    _check(lib().fl_device_memory(C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def cfl_dt(model, h, courant=0.4):

@@ -59,7 +59,7 @@ constexpr int prefetch_steps()
 #ifdef FL_PREFETCH
     return FL_PREFETCH;
 #else
-    return (CM == CM_BF16 || CM == CM_F16) ? 1 : 2;
+    return (CM == CM_BF16 || CM == CM_F16 || CM == CM_B16Q) ? 1 : 2;
 #endif
 }
 
@@ -95,7 +95,7 @@ __device__ __forceinline__ uint32_t bload32(__amdgpu_buffer_rsrc_t r, uint32_t v
 // loads stay in flight across barriers.
 template <int CM>
 struct CoefRec {
-    static constexpr int NW = (CM == CM_F32) ? 5 : 3;
+    static constexpr int NW = (CM == CM_F32 || CM == CM_B16Q) ? 5 : 3;
     uint32_t w[NW];
 };
 
@@ -117,6 +117,9 @@ __device__ __forceinline__ RecOff rec_offsets(uint32_t col, int fld, bool active
     } else if constexpr (CM == CM_F32R) {
         o.voff = active ? col * 24u + (uint32_t)fld * 12u : FL_OOB;
         o.voff2 = 0;
+    } else if constexpr (CM == CM_B16Q) {
+        o.voff = active ? col * 20u : FL_OOB;
+        o.voff2 = 0;
     } else {
         o.voff = active ? col * 20u + (uint32_t)fld * 8u : FL_OOB;
         o.voff2 = active ? col * 20u + 16u : FL_OOB;
@@ -127,7 +130,7 @@ __device__ __forceinline__ RecOff rec_offsets(uint32_t col, int fld, bool active
 template <int CM, int PCPOL>
 __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, const RecOff &o, CoefRec<CM> &c)
 {
-    if constexpr (CM == CM_F32) {
+    if constexpr (CM == CM_F32 || CM == CM_B16Q) {
         const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, o.voff, 0, 0);
         c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
         c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff, 16, 0);
@@ -178,6 +181,16 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, Coef &
         k.c[7] = __builtin_bit_cast(float, from_q(r.w[2]));
         k.c[8] = __builtin_bit_cast(float, from_q(r.w[3]));
         k.c[6] = k.c[9] = __builtin_bit_cast(float, r.w[4]);
+    } else if constexpr (CM == CM_B16Q) {
+        // angular coefficients BF16 (halfwords 0-5), velocity factors as CM_Q16R (words 3, 4)
+        k.c[0] = bf16_lo(r.w[0]); k.c[1] = bf16_hi(r.w[0]);
+        k.c[2] = bf16_lo(r.w[1]); k.c[3] = bf16_hi(r.w[1]);
+        k.c[4] = bf16_lo(r.w[2]); k.c[5] = bf16_hi(r.w[2]);
+        const uint32_t e = (r.w[3] >> 30) | ((r.w[4] >> 28) & 0xcu);
+        const float sc = __builtin_bit_cast(float, (e + 127u) << 23);  // 2^e
+        k.c[6] = k.c[9] = (float)((r.w[3] >> (fld ? 15 : 0)) & 0x7fffu) * sc;
+        k.c[7] = (float)(r.w[4] & 0x7fffu) * sc;
+        k.c[8] = (float)((r.w[4] >> 15) & 0x7fffu) * sc;
     } else if constexpr (CM == CM_BF16 || CM == CM_F16) {
         const uint32_t a0 = from_p(r.w[0]), a1 = from_p(r.w[1]);
         const uint32_t b0 = from_q(r.w[0]), b1 = from_q(r.w[1]);
@@ -508,7 +521,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             const float base = 2.0f * cur - vprev;
             float nw;
             if constexpr (CM == CM_F32 || CM == CM_F32R) nw = __builtin_fmaf(rhs * dts, a.dt, base);
-            else if constexpr (CM == CM_Q16R) nw = __builtin_fmaf(rhs, rq, base);
+            else if constexpr (CM == CM_Q16R || CM == CM_B16Q) nw = __builtin_fmaf(rhs, rq, base);
             else if constexpr (ISO) nw = __builtin_fmaf(rhs, a.dxxinv, base);
             else nw = base + rhs;
             if (z == rec_z) {
@@ -578,9 +591,9 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
 
 // Waves per SIMD the register allocator is asked for.  With 8-row tiles the LDS rings
 // (51 KB per workgroup) allow three workgroups per CU, with 12-row tiles (63 KB) two:
-// 6 waves per SIMD, <= 80 VGPRs, which the 16-bit and recompute forms meet.  The FP32 records keep two workgroups
-// (4 waves, 85 VGPRs): held to 80 they spill one value per step and run 7% slower at
-// 344^3.  4- and 16-row tiles are held to 4 waves per SIMD by their LDS.
+// 6 waves per SIMD, <= 80 VGPRs, which the 16-bit and recompute forms meet.  The 5-word
+// records (FP32, BF16 angular + block floating point) keep 4 waves: held to 80 VGPRs they
+// spill.  4- and 16-row tiles are held to 4 waves per SIMD by their LDS.
 // FL_STEP_WAVES overrides the 8- and 12-row value at build time.
 template <int TY, int CM>
 constexpr int step_waves()
@@ -588,7 +601,7 @@ constexpr int step_waves()
 #ifdef FL_STEP_WAVES
     return (TY == 8 || TY == 12) ? FL_STEP_WAVES : 4;
 #else
-    return ((TY == 8 || TY == 12) && CM != CM_F32) ? 6 : 4;
+    return ((TY == 8 || TY == 12) && CM != CM_F32 && CM != CM_B16Q) ? 6 : 4;
 #endif
 }
 
@@ -602,9 +615,9 @@ void tti_step(const StepArgs a)
         a.run[1 - a.par] = a.run[a.par] + 1;
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     if (t >= a.nblk) return;
-    const int tix = t % a.ntx;
-    const int rest = t / a.ntx;
-    step_tile<TY, CM, TWOPASS, ISO, HINTS ? 2 : 0>(a, ctl, tix, rest % a.nty, rest / a.nty, s);
+    const int tix = uni(t % a.ntx);
+    const int rest = uni(t / a.ntx);
+    step_tile<TY, CM, TWOPASS, ISO, HINTS ? 2 : 0>(a, ctl, tix, uni(rest % a.nty), uni(rest / a.nty), s);
 }
 
 // Shot of a batched run (device copy of the host descriptor).
@@ -655,46 +668,52 @@ constexpr int batch_waves()
 #ifdef FL_BATCH_WAVES
     return (TY == 8 || TY == 12) ? FL_BATCH_WAVES : 4;
 #else
-    return ((TY == 8 || TY == 12) && CM != CM_F32) ? 6 : 4;
+    return ((TY == 8 || TY == 12) && CM != CM_F32 && CM != CM_B16Q) ? 6 : 4;
 #endif
 }
+// The shot descriptors and the item table come in as read-only, non-aliased kernel
+// parameters as well: loads from them are then scalar (invariant over the kernel).
 template <int TY, int CM, bool ISO>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(batch_waves<TY, CM>())))
-void tti_step_batch(const BatchArgs b)
+void tti_step_batch(const BatchArgs b, const ShotDev *__restrict__ shots, const int *__restrict__ item_shot)
 {
     __shared__ StepLds<TY> s;
-    const int k = b.run[b.par];
+    const int k = uni(b.run[b.par]);
     if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) b.run[1 - b.par] = k + 1;
-    const int it = b.run[RUN_IT_BASE] + 1 + k;
-    const int src_n = b.run[RUN_SRC_N];
+    const int it = uni(b.run[RUN_IT_BASE]) + 1 + k;
+    const int src_n = uni(b.run[RUN_SRC_N]);
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     if (t >= b.nitems) return;
-    const ShotDev &sh = b.shots[__builtin_amdgcn_readfirstlane(b.item_shot[t])];
+    const ShotDev &sh = shots[uni(item_shot[t])];
     StepArgs a;
-    a.prev = sh.lv[b.parity];
-    a.cur = sh.lv[1 - b.parity];
-    a.coef = sh.coef;
+    a.prev = uni(sh.lv[b.parity]);
+    a.cur = uni(sh.lv[1 - b.parity]);
+    a.coef = uni(sh.coef);
     a.dvx = a.dvy = nullptr;
     a.dxxinv = b.dxxinv; a.dyyinv = b.dyyinv; a.dzzinv = b.dzzinv;
     a.dxinv = b.dxinv; a.dyinv = b.dyinv; a.dzinv = b.dzinv; a.dt = b.dt;
-    a.vq = sh.vq;
-    a.sx = sh.sx; a.sy = sh.sy; a.sz = sh.sz; a.bord = b.bord; a.px = sh.px;
-    a.ntx = sh.ntx; a.nty = sh.nty; a.zt = b.zt; a.nblk = sh.nitems; a.xorg = sh.xorg;
-    a.rec_iz = sh.rec_iz;
-    a.src_idx = sh.src_idx;
+    a.vq = uni(sh.vq);
+    a.sx = uni(sh.sx); a.sy = uni(sh.sy); a.sz = uni(sh.sz); a.bord = b.bord; a.px = uni(sh.px);
+    a.ntx = uni(sh.ntx); a.nty = uni(sh.nty); a.zt = b.zt; a.nblk = uni(sh.nitems); a.xorg = uni(sh.xorg);
+    a.rec_iz = uni(sh.rec_iz);
+    a.src_idx = uni(sh.src_idx);
     StepCtl ctl;
-    ctl.do_src = b.src_tab && (k + 1) < src_n;
-    ctl.src_add = ctl.do_src ? b.src_tab[k + 1] : 0.0f;
+    // the table holds src_n >= 1 samples: the clamped load needs no branch
+    ctl.do_src = (k + 1) < src_n;
+    const float s_next = uni(b.src_tab[min(k + 1, src_n - 1)]);
+    ctl.src_add = uni(ctl.do_src ? s_next : 0.0f);
     ctl.trace = nullptr;
-    if (sh.trace && b.rec_every > 0 && it % b.rec_every == 0) {
+    float *const trace = uni(sh.trace);
+    if (trace && b.rec_every > 0 && it % b.rec_every == 0) {
         const int slot = it / b.rec_every - 1;
         if (slot >= 0 && slot < b.rec_slots)
-            ctl.trace = sh.trace + (long long)slot * (sh.sx - 2 * b.bord) * (sh.sy - 2 * b.bord);
+            ctl.trace = trace + (long long)slot * (a.sx - 2 * b.bord) * (a.sy - 2 * b.bord);
     }
-    const int l = t - sh.item0;
-    const int tix = l % sh.ntx;
-    const int rest = l / sh.ntx;
-    step_tile<TY, CM, false, ISO, 0>(a, ctl, tix, rest % sh.nty, rest / sh.nty, s);
+    ctl.trace = uni(ctl.trace);
+    const int l = t - uni(sh.item0);
+    const int tix = uni(l % a.ntx);
+    const int rest = uni(l / a.ntx);
+    step_tile<TY, CM, false, ISO, 0>(a, ctl, tix, uni(rest % a.nty), uni(rest / a.nty), s);
 }
 
 }  // namespace fl

@@ -36,15 +36,18 @@ enum { FL_VARIANT_FUSED = 0, FL_VARIANT_DIRECT = 1, FL_VARIANT_TWOPASS = 2 };
  * block floating point (15-bit mantissas and one shared exponent), 16 times finer than
  * FP16 for the cell's largest factor; FP32_RECOMPUTE stores the 4 precomp.h velocity factors bit for bit and
  * theta, phi as FP32 turns, and rebuilds the angular coefficients in FP32 (24 bytes per
- * cell instead of 40).  Wavefields stay FP32.  The recompute forms and the 16-bit forms
- * run with the fused variant. */
+ * cell instead of 40); BF16_BFP stores the 6 angular coefficients in BF16 and the 4
+ * velocity factors (times dt^2) in the block floating point of Q16_RECOMPUTE, 20 bytes per
+ * cell like BF16 (whose 8-bit velocity factors are its main error).  Wavefields stay FP32.
+ * The recompute forms and the 16-bit forms run with the fused variant. */
 enum {
     FL_PREC_FP32 = 0,
     FL_PREC_BF16 = 1,
     FL_PREC_FP16 = 2,
     FL_PREC_FP16_RECOMPUTE = 3,
     FL_PREC_FP32_RECOMPUTE = 4,
-    FL_PREC_Q16_RECOMPUTE = 5
+    FL_PREC_Q16_RECOMPUTE = 5,
+    FL_PREC_BF16_BFP = 6
 };
 
 const char *fl_last_error(void);
@@ -52,7 +55,8 @@ int fl_status(void);      /* code of the last failed call, FL_OK after fl_clear_
 void fl_clear_error(void);
 
 /* Front-ends that call only driver.h select with the environment instead, read by
- * DRIVER_Initialize: FL_VARIANT=fused|direct|twopass, FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r|q16r,
+ * DRIVER_Initialize: FL_VARIANT=fused|direct|twopass,
+ * FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r|q16r|bf16q,
  * FL_STREAM_HINTS=0|1 (fl_set_stream_hints), FL_TUNE=<cache file> (autotune right after
  * initialisation). */
 int fl_set_variant(int variant);   /* takes effect at the next step */
@@ -118,6 +122,8 @@ int fl_get_derivative_volume(int k, float *host);
 
 int fl_device_name(char *buf, int len);
 long long fl_device_bytes(void);   /* device memory held by the back-end */
+/* Free and total memory of the device (hipMemGetInfo). */
+int fl_device_memory(long long *free_bytes, long long *total_bytes);
 /* Algorithmic HBM bytes per updated cell per step of the current configuration. */
 int fl_bytes_per_cell(void);
 

@@ -1,10 +1,10 @@
 """Mixed-precision coefficient storage and the batched multi-shot engine on the GPU.
 
-* BF16 / FP16 / FP16-recompute / block-floating-point-recompute / FP32-recompute coefficients against the FP64
-  oracle
+* BF16 / FP16 / FP16-recompute / block-floating-point-recompute / FP32-recompute / BF16 angular with
+  block-floating-point velocity (bf16q) coefficients against the FP64 oracle
   (errors printed and bounded; FP32 stays the 1e-5 drop-in);
 * every shot of a batch equals the same shot run alone through the drop-in driver
-  (same per-cell arithmetic: bitwise on normal floats), for FP32 and BF16 coefficients,
+  (same per-cell arithmetic: bit for bit), for FP32 and BF16 coefficients,
   with and without graph capture, including receiver traces.
 """
 import numpy as np
@@ -15,13 +15,8 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 D = (12.5, 12.5, 12.5)
-TINY = np.finfo(np.float32).tiny
 
 
-def normal_part(a):
-    a = a.copy()
-    a[np.abs(a) < TINY] = 0.0
-    return a
 
 
 @pytest.fixture(scope="module")
@@ -39,8 +34,13 @@ def mixed_case(fl, oracle):
     return dict(m=m, shape=shape, dt=dt, rec=rec, isrc=isrc, st=st, ref=ref)
 
 
-@pytest.mark.parametrize("prec,bound", [("fp32", 5e-5), ("fp32r", 5e-5), ("fp16", 2e-3), ("fp16r", 3e-3),
-                                        ("q16r", 2e-4), ("bf16", 3e-2)])
+# Bounds are the claims: 1e-3 (the mixed-precision target) for FP16 and FP16 recompute,
+# 2e-4 for block floating point.  The FP32 forms are bounded at 2e-5 against FP64 here
+# because the FP32 reference path itself is 9e-6 from FP64 on this grid (their parity bar,
+# 1e-5 against the FP32 oracle, is tested elsewhere).  BF16 misses the target: it is bounded
+# at its measured error (2.8e-3 on traces here) and reported as failing.
+@pytest.mark.parametrize("prec,bound", [("fp32", 2e-5), ("fp32r", 2e-5), ("fp16", 1e-3), ("fp16r", 1e-3),
+                                        ("q16r", 2e-4), ("bf16q", 1e-3), ("bf16", 5e-3)])
 def test_precision_error_vs_fp64_oracle(fl, mixed_case, prec, bound):
     c = mixed_case
     d = fl.Driver(variant="fused", precision=prec)
@@ -56,7 +56,7 @@ def test_precision_error_vs_fp64_oracle(fl, mixed_case, prec, bound):
     assert e_pc <= bound and e_tr <= bound
 
 
-@pytest.mark.parametrize("prec,bound", [("fp16r", 3e-3), ("q16r", 2e-4), ("bf16", 3e-2)])
+@pytest.mark.parametrize("prec,bound", [("fp16r", 1e-3), ("q16r", 2e-4), ("bf16q", 1e-3), ("bf16", 2e-3)])
 def test_anisotropic_precision_error_vs_fp64_oracle(fl, oracle, prec, bound):
     """The 16-bit forms on a grid with dx != dy != dz (spacing applied per derivative)."""
     d3 = (12.5, 10.0, 15.0)
@@ -115,7 +115,7 @@ def test_batch_equals_single_shots(fl, prec, chunk):
             d.finalize()
         for x, y in zip(got[k], ref):
             assert np.abs(y).max() > 0
-            assert np.array_equal(normal_part(x), normal_part(y)), (k, rel_l2(x, y))
+            assert np.array_equal(x, y), (k, rel_l2(x, y))
 
 
 @pytest.mark.parametrize("prec", ["fp32", "q16r"])
@@ -139,7 +139,7 @@ def test_batch_equals_single_shots_anisotropic(fl, prec):
         finally:
             d.finalize()
         assert np.abs(ref).max() > 0
-        assert np.array_equal(normal_part(got[k]), normal_part(ref)), (k, rel_l2(got[k], ref))
+        assert np.array_equal(got[k], ref), (k, rel_l2(got[k], ref))
 
 
 def test_batch_continues_across_runs_and_tiles(fl):
@@ -161,7 +161,7 @@ def test_batch_continues_across_runs_and_tiles(fl):
     finally:
         b.destroy()
     for x, y in zip(got, ref):
-        assert np.array_equal(normal_part(x), normal_part(y))
+        assert np.array_equal(x, y)
 
 
 def test_batch_oracle_parity(fl, oracle):

@@ -147,10 +147,15 @@ __device__ __forceinline__ long long uni(long long v)
     const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
     return (long long)(((unsigned long long)hi << 32) | lo);
 }
+// Pointers into device memory: rebuilt as global (address space 1) pointers, so that
+// accesses through them stay global_* instructions.  A generic pointer would make them
+// flat_* instructions, which may complete out of order with the buffer loads and make the
+// compiler drain the whole load queue (s_waitcnt vmcnt(0)) before the next use of any load.
 template <class T>
 __device__ __forceinline__ T *uni(T *p)
 {
-    return (T *)(uintptr_t)uni((long long)(uintptr_t)p);
+    typedef __attribute__((address_space(1))) T global_t;
+    return (T *)(global_t *)(uintptr_t)uni((long long)(uintptr_t)p);
 }
 
 // Control block of a multi-step run (StepArgs.run, BatchArgs.run).

@@ -25,8 +25,7 @@ extras (skipped with --quick), each timed over its whole run with inputs residen
 graphs replay a graph captured by an untimed first run:
   twopass   the der1der1 two-pass variant on the same grid (104 algorithmic B/sample)
   fp32_graph / mixed   FP32, the FP32 in-kernel recompute (fp32r: FP32 arithmetic, 24-byte
-            records), BF16 / FP16 coefficients, BF16 angular coefficients with block
-            floating-point velocity factors (bf16q), the FP16 in-kernel recompute and its
+            records), BF16 / FP16 coefficients, the FP16 in-kernel recompute and its
             block floating-point variant (q16r) on the same grid, receivers on plane z=28
             every 4 steps, 50-step graph chunks
   rel_l2_vs_fp64   relative L2 (in FP64) of the receiver traces and the final pc against
@@ -34,7 +33,7 @@ graphs replay a graph captured by an untimed first run:
             bitwise to the FP64 oracle by the tests) on the same grid and steps
   batched   32 shots on uneven (128..256)^2 x 320 apertures cut from one seeded
             512x512x320 model (absorb 16), 800 steps in 50-step graphs, autotuned
-            tiles, FP32, BF16, bf16q, FP16-recompute and block floating-point-recompute coefficients;
+            tiles, FP32, BF16, FP16-recompute and block floating-point-recompute coefficients;
             receivers every 4 steps, their relative L2 against the FP64 device reference
             for the first two shots
 """
@@ -339,8 +338,7 @@ def main():
     ap.add_argument("--nx", type=int, default=344)
     ap.add_argument("--absorb", type=int, default=16)
     ap.add_argument("--variant", default="fused", choices=["fused", "direct", "twopass"])
-    ap.add_argument("--precision", default="fp32",
-                    choices=["fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r", "bf16q"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r"])
     ap.add_argument("--tiles", default="auto", help="ty,zt[,stream hints 0/1] or 'auto' (autotuner)")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
@@ -389,13 +387,13 @@ def main():
                   "fp32_graph": graph_run(F, np, model, shape, dt, isrc, "fused", "fp32", args.steps,
                                           rec_iz, "auto", ref),
                   "mixed": {}}
-        for prec in ("fp32r", "bf16", "bf16q", "fp16", "fp16r", "q16r"):
+        for prec in ("fp32r", "bf16", "fp16", "fp16r", "q16r"):
             r = graph_run(F, np, model, shape, dt, isrc, "fused", prec, args.steps, rec_iz, "auto", ref)
             r["vs_fp32_dropin"] = round(r["value"] / value, 3)
             extras["mixed"][prec] = r
         del ref
-        bt = batched(F, np, ("fp32", "bf16", "bf16q", "fp16r", "q16r"), args.batch_steps, args.shots)
-        for prec in ("fp32", "bf16", "bf16q", "fp16r", "q16r"):
+        bt = batched(F, np, ("fp32", "bf16", "fp16r", "q16r"), args.batch_steps, args.shots)
+        for prec in ("fp32", "bf16", "fp16r", "q16r"):
             bt[prec]["vs_fp32_dropin"] = round(bt[prec]["value"] / value, 3)
         extras["batched"] = bt
 

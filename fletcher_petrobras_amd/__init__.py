@@ -26,7 +26,7 @@ CLI_PATH = os.path.join(_HERE, "bin", "fletcher")
 BORD = 4  # stencil halo (main.c:21)
 FORM = {"ISO": 0, "VTI": 1, "TTI": 2}
 VARIANT = {"fused": 0, "direct": 1, "twopass": 2}
-PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp16r": 3, "fp32r": 4, "q16r": 5, "bf16q": 6}
+PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp16r": 3, "fp32r": 4, "q16r": 5}
 FIELD = {"pp": 0, "pc": 1, "qp": 2, "qc": 3}
 COEF_NAMES = ("ch1dxx", "ch1dyy", "ch1dzz", "ch1dxy", "ch1dyz", "ch1dxz",
               "v2px", "v2pz", "v2sz", "v2pn")
@@ -123,7 +123,6 @@ def lib():
         "fl_pack_coefs16": (None, [C.c_int64] + [_f32p] * 6 + [d, i, C.POINTER(vp)]),
         "fl_pack_recompute16": (None, [C.c_int64] + [_f32p] * 6 + [d, C.POINTER(vp)]),
         "fl_pack_recompute32": (None, [C.c_int64] + [_f32p] * 6 + [C.POINTER(vp)]),
-        "fl_pack_coefs_b16q": (None, [C.c_int64] + [_f32p] * 6 + [d, i, C.POINTER(vp)]),
         "fl_recompute_q16_base": (i, [C.c_int64] + [_f32p] * 4 + [d]),
         "fl_pack_recompute_q16": (None, [C.c_int64] + [_f32p] * 6 + [d, i, C.POINTER(vp)]),
         "fl_extract_window": (None, [_f32p] + [i] * 9 + [_f32p]),

@@ -71,7 +71,7 @@ struct Device {
     int lc = 1;             // level holding pc, qc; lv[1 - lc] holds pp, qp
     void *coef = nullptr;   // coefficient records, pitched like the levels
     int cm = CM_F32;
-    float vq = 0;           // CM_Q16R, CM_B16Q: 2^ebase
+    float vq = 0;           // CM_Q16R: 2^ebase
     float *dvx = nullptr, *dvy = nullptr;  // two-pass (pDx, qDx), (pDy, qDy) pairs
     float *trace = nullptr;
     int rec_iz = -1, rec_every = 0, rec_slots = 0;
@@ -163,7 +163,7 @@ extern "C" int fl_bytes_per_cell(void)
 // values (FP32 or the 16-bit forms, or the recompute inputs) in the slot order
 // rec_order32 / rec_order16 / rec_order_r32 / rec_order_r16 (record of
 // coef_count * coef_elem bytes); z slabs of at most 64 MiB per array on the host.
-// ebase: the CM_Q16R / CM_B16Q exponent base (fl_recompute_q16_base over the whole grid).
+// ebase: the CM_Q16R exponent base (fl_recompute_q16_base over the whole grid).
 static int upload_coefs(const float *vpz, const float *vsv, const float *eps, const float *del,
                         const float *phi, const float *theta, float dt, int cm, void *dst, int sx,
                         int sy, int sz, int px, int ebase = 0)
@@ -202,9 +202,6 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
             else if (cm == CM_Q16R)
                 fl_pack_recompute_q16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
                                       ebase, c);
-            else if (cm == CM_B16Q)
-                fl_pack_coefs_b16q(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
-                                   ebase, c);
             else
                 fl_pack_coefs16(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, dt,
                                 cm == CM_BF16, c);
@@ -222,7 +219,7 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
             }
             const uint16_t *co[10];
             for (int k = 0; k < nk; k++)
-                co[k] = c[(cm == CM_F16R || cm == CM_Q16R) ? rec_order_r16(k) : cm == CM_B16Q ? k : rec_order16(k)];
+                co[k] = c[(cm == CM_F16R || cm == CM_Q16R) ? rec_order_r16(k) : rec_order16(k)];
             fl_interleave16(m, nk, co, r16.data());
             HIPCHK(hipMemcpy2D((char *)dst + (size_t)z0 * sy * px * rec, px * rec, r16.data(), sx * rec,
                                sx * rec, (size_t)sy * (z1 - z0), hipMemcpyHostToDevice));
@@ -234,11 +231,11 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
 // ------------------------------------------------------------------ launch plumbing
 // Workgroups of the step kernel resident on one CU: the LDS rings (6 staged planes and 5
 // Dx planes of ty+8 rows) against 160 KB, and the waves per SIMD the kernel is built for
-// (step_waves: 4 for the 5-word records, 6 for the other forms with 8- or 12-row tiles).
+// (step_waves: 4 for FP32, 6 for the other forms with 8- or 12-row tiles).
 static int resident_per_cu(int ty, int cm)
 {
     const long long lds = (6ll * (ty + 8) * 80 + 5ll * (ty + 8) * 64) * 4;
-    const int waves = (cm != CM_F32 && cm != CM_B16Q && (ty == 8 || ty == 12)) ? 6 : 4;
+    const int waves = (cm != CM_F32 && (ty == 8 || ty == 12)) ? 6 : 4;
     return std::max(1, (int)std::min<long long>((160ll << 10) / lds, 4 * waves / ty));
 }
 
@@ -534,7 +531,7 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
     const size_t cbytes = (size_t)coef_count(D.cm) * D.np * coef_elem(D.cm);
     if ((rc = dev_alloc(&D.coef, cbytes, &D.bytes))) { free_device(); return rc; }
     int ebase = 0;
-    if (D.cm == CM_Q16R || D.cm == CM_B16Q) {
+    if (D.cm == CM_Q16R) {
         ebase = fl_recompute_q16_base(D.n, vpz, vsv, eps, del, dt);
         D.vq = std::ldexp(1.0f, ebase);
     }
@@ -549,7 +546,7 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
 
 // Selection through the environment for front-ends that only call driver.h (the
 // reference's main.c/model.c): FL_VARIANT = fused|direct|twopass, FL_PRECISION =
-// fp32|bf16|fp16|fp16r|fp32r|q16r|bf16q, FL_STREAM_HINTS = 0|1, applied at DRIVER_Initialize when set;
+// fp32|bf16|fp16|fp16r|fp32r|q16r, FL_STREAM_HINTS = 0|1, applied at DRIVER_Initialize when set;
 // FL_TUNE = cache file, the autotuner runs (or reads the cache) right after it.
 static int env_pick(const char *name, const char *const *opts, int n)
 {
@@ -569,9 +566,9 @@ extern "C" void DRIVER_Initialize(const int sx, const int sy, const int sz, cons
                                   float *__restrict__ qc)
 {
     static const char *const variants[] = {"fused", "direct", "twopass"};
-    static const char *const precs[] = {"fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r", "bf16q"};
+    static const char *const precs[] = {"fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r"};
     static const char *const onoff[] = {"0", "1"};
-    const int v = env_pick("FL_VARIANT", variants, 3), p = env_pick("FL_PRECISION", precs, 7);
+    const int v = env_pick("FL_VARIANT", variants, 3), p = env_pick("FL_PRECISION", precs, 6);
     const int h = env_pick("FL_STREAM_HINTS", onoff, 2);
     if (v == -2 || p == -2 || h == -2) {
         fail(FL_ERR_ARG, "FL_VARIANT / FL_PRECISION / FL_STREAM_HINTS hold an unknown value");
@@ -664,7 +661,7 @@ extern "C" int fl_get_variant(void) { return g_variant; }
 
 extern "C" int fl_set_precision(int p)
 {
-    if (p < FL_PREC_FP32 || p > FL_PREC_BF16_BFP) return fail(FL_ERR_ARG, "unknown precision %d", p);
+    if (p < FL_PREC_FP32 || p > FL_PREC_Q16_RECOMPUTE) return fail(FL_ERR_ARG, "unknown precision %d", p);
     g_precision = p;
     return FL_OK;
 }

@@ -52,7 +52,6 @@ int FL_CAT(launch_grid_, FL_TY)(const StepArgs &a, int cm, bool twopass, bool is
     case CM_F16R: zm_form<CM_F16R>(a, iso, hints, s); return 0;
     case CM_F32R: zm_form<CM_F32R>(a, iso, hints, s); return 0;
     case CM_Q16R: zm_form<CM_Q16R>(a, iso, hints, s); return 0;
-    case CM_B16Q: zm_form<CM_B16Q>(a, iso, hints, s); return 0;
     }
     return -1;
 }
@@ -79,7 +78,6 @@ int FL_CAT(launch_batch_, FL_TY)(const BatchArgs &a, int grid, int cm, bool iso,
     case CM_F16R: bt<CM_F16R>(a, grid, iso, s); return 0;
     case CM_F32R: bt<CM_F32R>(a, grid, iso, s); return 0;
     case CM_Q16R: bt<CM_Q16R>(a, grid, iso, s); return 0;
-    case CM_B16Q: bt<CM_B16Q>(a, grid, iso, s); return 0;
     }
     return -1;
 }

@@ -53,14 +53,12 @@ enum CoefMode : int {
     CM_F32R = 4,  // 4 FP32 velocity factors (precomp.h values) and theta, phi as FP32 turns
     CM_Q16R = 5,  // 4 velocity factors (times dt^2) in per-cell block floating point (15-bit
                   // mantissas, shared exponent), theta, phi as u16 turns
-    CM_B16Q = 6,  // the 6 angular coefficients in BF16, the 4 velocity factors (times dt^2)
-                  // in per-cell block floating point as CM_Q16R
 };
 
 // Bytes of one cell's coefficient record.
 __host__ __device__ constexpr int rec_bytes(int cm)
 {
-    return cm == CM_F32 ? 40 : (cm == CM_F16R || cm == CM_Q16R) ? 12 : cm == CM_F32R ? 24 : 20;  // BF16, FP16, B16Q: 20
+    return cm == CM_F32 ? 40 : (cm == CM_F16R || cm == CM_Q16R) ? 12 : cm == CM_F32R ? 24 : 20;
 }
 
 // Order of the precomp.h coefficients (0 ch1dxx, 1 ch1dyy, 2 ch1dzz, 3 ch1dxy, 4 ch1dyz,
@@ -74,9 +72,6 @@ __host__ __device__ constexpr int rec_bytes(int cm)
 //   FP16 and block floating-point recompute (12 B): words (v2px, v2pn), (v2pz, v2sz),
 //                          (theta, phi), loaded whole by both lanes;
 //   FP32 recompute (24 B): p lane {v2px, v2pz, theta}, q lane {v2pn, v2sz, phi}.
-// BF16 angular + block floating-point velocity (20 B): halfwords 0-5 the angular
-// coefficients in precomp order (BF16), then the words of the CM_Q16R velocity factors,
-// (v2px, v2pn) and (v2pz, v2sz); both lanes load the whole record.
 __host__ __device__ constexpr int rec_order32(int s)
 {
     constexpr int o[10] = {0, 1, 2, 3, 6, 4, 5, 7, 8, 9};

@@ -59,7 +59,7 @@ constexpr int prefetch_steps()
 #ifdef FL_PREFETCH
     return FL_PREFETCH;
 #else
-    return (CM == CM_BF16 || CM == CM_F16 || CM == CM_B16Q) ? 1 : 2;
+    return (CM == CM_BF16 || CM == CM_F16) ? 1 : 2;
 #endif
 }
 
@@ -95,7 +95,7 @@ __device__ __forceinline__ uint32_t bload32(__amdgpu_buffer_rsrc_t r, uint32_t v
 // loads stay in flight across barriers.
 template <int CM>
 struct CoefRec {
-    static constexpr int NW = (CM == CM_F32 || CM == CM_B16Q) ? 5 : 3;
+    static constexpr int NW = (CM == CM_F32) ? 5 : 3;
     uint32_t w[NW];
 };
 
@@ -117,9 +117,6 @@ __device__ __forceinline__ RecOff rec_offsets(uint32_t col, int fld, bool active
     } else if constexpr (CM == CM_F32R) {
         o.voff = active ? col * 24u + (uint32_t)fld * 12u : FL_OOB;
         o.voff2 = 0;
-    } else if constexpr (CM == CM_B16Q) {
-        o.voff = active ? col * 20u : FL_OOB;
-        o.voff2 = 0;
     } else {
         o.voff = active ? col * 20u + (uint32_t)fld * 8u : FL_OOB;
         o.voff2 = active ? col * 20u + 16u : FL_OOB;
@@ -130,7 +127,7 @@ __device__ __forceinline__ RecOff rec_offsets(uint32_t col, int fld, bool active
 template <int CM, int PCPOL>
 __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, const RecOff &o, CoefRec<CM> &c)
 {
-    if constexpr (CM == CM_F32 || CM == CM_B16Q) {
+    if constexpr (CM == CM_F32) {
         const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, o.voff, 0, 0);
         c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
         c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff, 16, 0);
@@ -181,16 +178,6 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, Coef &
         k.c[7] = __builtin_bit_cast(float, from_q(r.w[2]));
         k.c[8] = __builtin_bit_cast(float, from_q(r.w[3]));
         k.c[6] = k.c[9] = __builtin_bit_cast(float, r.w[4]);
-    } else if constexpr (CM == CM_B16Q) {
-        // angular coefficients BF16 (halfwords 0-5), velocity factors as CM_Q16R (words 3, 4)
-        k.c[0] = bf16_lo(r.w[0]); k.c[1] = bf16_hi(r.w[0]);
-        k.c[2] = bf16_lo(r.w[1]); k.c[3] = bf16_hi(r.w[1]);
-        k.c[4] = bf16_lo(r.w[2]); k.c[5] = bf16_hi(r.w[2]);
-        const uint32_t e = (r.w[3] >> 30) | ((r.w[4] >> 28) & 0xcu);
-        const float sc = __builtin_bit_cast(float, (e + 127u) << 23);  // 2^e
-        k.c[6] = k.c[9] = (float)((r.w[3] >> (fld ? 15 : 0)) & 0x7fffu) * sc;
-        k.c[7] = (float)(r.w[4] & 0x7fffu) * sc;
-        k.c[8] = (float)((r.w[4] >> 15) & 0x7fffu) * sc;
     } else if constexpr (CM == CM_BF16 || CM == CM_F16) {
         const uint32_t a0 = from_p(r.w[0]), a1 = from_p(r.w[1]);
         const uint32_t b0 = from_q(r.w[0]), b1 = from_q(r.w[1]);
@@ -521,7 +508,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             const float base = 2.0f * cur - vprev;
             float nw;
             if constexpr (CM == CM_F32 || CM == CM_F32R) nw = __builtin_fmaf(rhs * dts, a.dt, base);
-            else if constexpr (CM == CM_Q16R || CM == CM_B16Q) nw = __builtin_fmaf(rhs, rq, base);
+            else if constexpr (CM == CM_Q16R) nw = __builtin_fmaf(rhs, rq, base);
             else if constexpr (ISO) nw = __builtin_fmaf(rhs, a.dxxinv, base);
             else nw = base + rhs;
             if (z == rec_z) {
@@ -591,9 +578,9 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
 
 // Waves per SIMD the register allocator is asked for.  With 8-row tiles the LDS rings
 // (51 KB per workgroup) allow three workgroups per CU, with 12-row tiles (63 KB) two:
-// 6 waves per SIMD, <= 80 VGPRs, which the 16-bit and recompute forms meet.  The 5-word
-// records (FP32, BF16 angular + block floating point) keep 4 waves: held to 80 VGPRs they
-// spill.  4- and 16-row tiles are held to 4 waves per SIMD by their LDS.
+// 6 waves per SIMD, <= 80 VGPRs, which the 16-bit and recompute forms meet.  The FP32
+// records keep 4 waves (83 VGPRs): a 6-wave build spills only outside the steady loop but
+// measured no faster at 344^3.  4- and 16-row tiles are held to 4 waves per SIMD by their LDS.
 // FL_STEP_WAVES overrides the 8- and 12-row value at build time.
 template <int TY, int CM>
 constexpr int step_waves()
@@ -601,7 +588,7 @@ constexpr int step_waves()
 #ifdef FL_STEP_WAVES
     return (TY == 8 || TY == 12) ? FL_STEP_WAVES : 4;
 #else
-    return ((TY == 8 || TY == 12) && CM != CM_F32 && CM != CM_B16Q) ? 6 : 4;
+    return ((TY == 8 || TY == 12) && CM != CM_F32) ? 6 : 4;
 #endif
 }
 
@@ -668,7 +655,7 @@ constexpr int batch_waves()
 #ifdef FL_BATCH_WAVES
     return (TY == 8 || TY == 12) ? FL_BATCH_WAVES : 4;
 #else
-    return ((TY == 8 || TY == 12) && CM != CM_F32 && CM != CM_B16Q) ? 6 : 4;
+    return ((TY == 8 || TY == 12) && CM != CM_F32) ? 6 : 4;
 #endif
 }
 // The shot descriptors and the item table come in as read-only, non-aliased kernel

@@ -6,7 +6,7 @@
  * reference's report lines (main.c:112-119, model.c:130-132).
  *
  * Back-end options come from the environment (the reference has none):
- *   FL_VARIANT=fused|direct|twopass   FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r|q16r|bf16q
+ *   FL_VARIANT=fused|direct|twopass   FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r|q16r
  *   FL_RSF=<name>  dump pc at the start and every 0.01 s of simulated time as <name>.rsf, as
  *                  der1der1/main.c:273-300 and model.c:97-102 do (utils.c:71-205 layout)
  *   FL_TUNE=<cache file>  autotune tiles for this grid at DRIVER_Initialize
@@ -55,9 +55,9 @@ int main(int argc, char **argv)
         return 2;
     }
     static const char *variants[] = {"fused", "direct", "twopass"};
-    static const char *precs[] = {"fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r", "bf16q"};
+    static const char *precs[] = {"fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r"};
     fl_set_variant(env_choice("FL_VARIANT", variants, 3, 0));
-    fl_set_precision(env_choice("FL_PRECISION", precs, 7, 0));
+    fl_set_precision(env_choice("FL_PRECISION", precs, 6, 0));
 
     const int sx = nx + 2 * bord + 2 * absorb, sy = ny + 2 * bord + 2 * absorb, sz = nz + 2 * bord + 2 * absorb;
     const int st = fl_num_steps(tmax, dt);

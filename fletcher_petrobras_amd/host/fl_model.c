@@ -396,34 +396,6 @@ void fl_pack_recompute_q16(int64_t n, const float *vpz, const float *vsv, const 
     }
 }
 
-void fl_pack_coefs_b16q(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
-                        const float *delta, const float *phi, const float *theta, double dt, int ebase,
-                        uint16_t *const *out)
-{
-    const double dt2 = dt * dt;
-#pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < n; i++) {
-        /* the six angular coefficients of fl_precompute_f64 rounded once to BF16 */
-        const double st = sin((double)theta[i]), ct = cos((double)theta[i]);
-        const double s2t = sin(2.0 * theta[i]);
-        const double sp = sin((double)phi[i]), cp = cos((double)phi[i]);
-        const double s2p = sin(2.0 * phi[i]);
-        const double c[6] = {st * st * cp * cp, st * st * sp * sp, ct * ct, st * st * s2p, s2t * sp, s2t * cp};
-        for (int k = 0; k < 6; k++) out[k][i] = fl_f64_to_bf16(c[k]);
-        /* the velocity factors times dt^2 as in fl_pack_recompute_q16 */
-        double f[4];
-        uint32_t q[4];
-        q16_factors(i, vpz, vsv, epsilon, delta, dt2, f);
-        const uint32_t e = (uint32_t)q16_block(f, ebase, q);
-        const uint32_t w0 = q[0] | (q[3] << 15) | ((e & 3u) << 30);
-        const uint32_t w1 = q[1] | (q[2] << 15) | ((e >> 2) << 30);
-        out[6][i] = (uint16_t)(w0 & 0xffffu);
-        out[7][i] = (uint16_t)(w0 >> 16);
-        out[8][i] = (uint16_t)(w1 & 0xffffu);
-        out[9][i] = (uint16_t)(w1 >> 16);
-    }
-}
-
 void fl_pack_recompute32(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, float *const *out)
 {

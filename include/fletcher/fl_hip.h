@@ -36,18 +36,15 @@ enum { FL_VARIANT_FUSED = 0, FL_VARIANT_DIRECT = 1, FL_VARIANT_TWOPASS = 2 };
  * block floating point (15-bit mantissas and one shared exponent), 16 times finer than
  * FP16 for the cell's largest factor; FP32_RECOMPUTE stores the 4 precomp.h velocity factors bit for bit and
  * theta, phi as FP32 turns, and rebuilds the angular coefficients in FP32 (24 bytes per
- * cell instead of 40); BF16_BFP stores the 6 angular coefficients in BF16 and the 4
- * velocity factors (times dt^2) in the block floating point of Q16_RECOMPUTE, 20 bytes per
- * cell like BF16 (whose 8-bit velocity factors are its main error).  Wavefields stay FP32.
- * The recompute forms and the 16-bit forms run with the fused variant. */
+ * cell instead of 40).  Wavefields stay FP32.  The recompute forms and the 16-bit forms
+ * run with the fused variant. */
 enum {
     FL_PREC_FP32 = 0,
     FL_PREC_BF16 = 1,
     FL_PREC_FP16 = 2,
     FL_PREC_FP16_RECOMPUTE = 3,
     FL_PREC_FP32_RECOMPUTE = 4,
-    FL_PREC_Q16_RECOMPUTE = 5,
-    FL_PREC_BF16_BFP = 6
+    FL_PREC_Q16_RECOMPUTE = 5
 };
 
 const char *fl_last_error(void);
@@ -56,7 +53,7 @@ void fl_clear_error(void);
 
 /* Front-ends that call only driver.h select with the environment instead, read by
  * DRIVER_Initialize: FL_VARIANT=fused|direct|twopass,
- * FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r|q16r|bf16q,
+ * FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r|q16r,
  * FL_STREAM_HINTS=0|1 (fl_set_stream_hints), FL_TUNE=<cache file> (autotune right after
  * initialisation). */
 int fl_set_variant(int variant);   /* takes effect at the next step */

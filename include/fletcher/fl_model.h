@@ -83,13 +83,6 @@ void fl_pack_recompute_q16(int64_t n, const float *vpz, const float *vsv, const 
  * fl_precompute_f32 (precomp.h:53-58) forms them, and theta, phi in turns reduced to
  * [-1/2, 1/2] (FP64, rounded once to FP32); the six angular coefficients are rebuilt
  * from the angles per cell in FP32. */
-/* BF16 angular coefficients with block floating-point velocity factors: halfwords 0-5 the
- * angular coefficients of fl_precompute_f64 (precomp order) in BF16, halfwords 6-9 the
- * words (v2px, v2pn) and (v2pz, v2sz) times dt^2 as fl_pack_recompute_q16 forms them
- * (exponent base ebase from fl_recompute_q16_base). */
-void fl_pack_coefs_b16q(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
-                        const float *delta, const float *phi, const float *theta, double dt, int ebase,
-                        uint16_t *const *out);
 void fl_pack_recompute32(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, float *const *out);
 /* Cell records: dst[i*nk + k] = src[k][i] (the device keeps the coefficients of a cell

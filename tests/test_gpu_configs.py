@@ -126,8 +126,7 @@ def headline(fl):
 # bf16 is listed at its measured error: it misses the 1e-3 target (8-bit significands)
 @pytest.mark.parametrize("prec,tiles,bound", [("fp32", (12, 188), TOL), ("fp32r", (12, 94), TOL),
                                               ("fp16", (12, 94), TOL16), ("fp16r", (12, 94), TOL16),
-                                              ("q16r", (12, 94), TOL16), ("bf16q", (12, 94), TOL16),
-                                              ("bf16", (12, 94), 2e-2)])
+                                              ("q16r", (12, 94), TOL16), ("bf16", (12, 94), 2e-2)])
 def test_headline_grid_vs_fp64(fl, drv, headline, prec, tiles, bound):
     c = headline
     d = drv(variant="fused", precision=prec, tiles=tiles)

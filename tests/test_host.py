@@ -438,28 +438,3 @@ def test_literal_config1_model_matches_reference_backend(fl, oracle):
     assert np.array_equal(r["pc"], R.fetch("pc")) and np.array_equal(r["qc"], R.fetch("qc"))
     assert float(np.abs(r["pc"]).max()) > 0
 
-
-def test_pack_coefs_b16q(fl):
-    """BF16 angular coefficients with block floating-point velocity factors: halfwords 0-5 are
-    the FP64 angular coefficients rounded once to BF16 (the BF16 form's own encoding, checked
-    against numpy's float32 truncation of the correctly rounded value), halfwords 6-9 the
-    velocity words of the block floating-point recompute form, bit for bit."""
-    m = fl.model_gradient((16, 14, 12), 4, seed=11)
-    n = m[0].size
-    dt = 8.0e-4
-    ebase = fl.lib().fl_recompute_q16_base(n, m[0], m[1], m[2], m[3], dt)
-    out = [np.empty(m[0].shape, np.uint16) for _ in range(10)]
-    fl.lib().fl_pack_coefs_b16q(n, *m, dt, ebase, (C.c_void_p * 10)(*[o.ctypes.data for o in out]))
-    q = [np.empty(m[0].shape, np.uint16) for _ in range(6)]
-    fl.lib().fl_pack_recompute_q16(n, *m, dt, ebase, (C.c_void_p * 6)(*[o.ctypes.data for o in q]))
-    assert np.array_equal(out[6], q[0]) and np.array_equal(out[7], q[3])
-    assert np.array_equal(out[8], q[1]) and np.array_equal(out[9], q[2])
-    c64 = fl.precompute(m, "f64")
-    for k in range(6):
-        got = (out[k].astype(np.uint32) << 16).view(np.float32).astype(np.float64)
-        ref = c64[k]
-        # BF16 keeps 8 significant bits: within half an ulp of 2^-8 relative, and exact at 0
-        tol = np.ldexp(1.0, np.frexp(np.abs(ref))[1] - 9)
-        assert np.all(np.abs(got - ref) <= tol * (1 + 1e-12)), k
-        assert np.array_equal(out[k], np.array([fl.lib().fl_f64_to_bf16(float(v)) for v in ref.ravel()],
-                                               np.uint16).reshape(ref.shape)), k

@@ -77,6 +77,7 @@ def lib():
         "fl_get_traces": (i, [_f32p, i]),
         "fl_run_steps": (i, [i, i, i, _f32p, i]),
         "fl_reset_fields": (i, []),
+        "fl_run_steps_static": (i, [i, i, i, _f32p]),
         "fl_synchronize": (i, []),
         "fl_set_timing": (i, [i]),
         "fl_kernel_times": (i, [C.POINTER(d), C.POINTER(i)]),

@@ -471,6 +471,32 @@ __global__ void set_run(int *run, int it_base, int src_n)
     }
 }
 
+// ------------------------------------------------------------------ allocation-free runs
+// Scratch of fl_run_steps_static: the run control block (RUN_WORDS ints) at offset 0 and up
+// to FL_STATIC_MAX_STEPS source samples at offset 256.  Each GPU holds its own copy at its
+// own address until the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returned for that device (g_scratch_dev), never through the name.
+#define FL_STATIC_MAX_STEPS 65536
+#define FL_STATIC_SRC_OFF 256
+static __device__ __align__(256) unsigned char scratch[FL_STATIC_SRC_OFF + 4 * FL_STATIC_MAX_STEPS];
+#define FL_MAX_DEVICES 64
+static char *g_scratch_dev[FL_MAX_DEVICES];
+
+// Resolves the scratch array of the current device once (called by DRIVER_Initialize).
+static int resolve_scratch()
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= FL_MAX_DEVICES) return fail(FL_ERR_HIP, "device %d beyond the scratch table", dev);
+    if (g_scratch_dev[dev]) return FL_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(FL_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = (char *)p;
+    return FL_OK;
+}
+
 // ------------------------------------------------------------------ driver.h
 static void free_device()
 {
@@ -503,6 +529,7 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
     if (max_zt(pitch_of(sx), sy) < 1)
         return fail(FL_ERR_ARG, "xy plane %dx%d too large for 32-bit plane offsets", sx, sy);
     int rc = ensure_stream();
+    if (!rc) rc = resolve_scratch();
     if (rc) return rc;
     free_device();
     const long long n = (long long)sx * sy * sz;
@@ -878,6 +905,60 @@ extern "C" int fl_run_steps(int it0, int nsteps, int iSource, const float *src, 
         int rc = launch_steps(step_args(k), g_tiles.ty, D.cm, false, g_stream);
         if (rc) return rc;
         swap_levels();
+    }
+    HIPCHK(hipGetLastError());
+    if (g_sync) HIPCHK(hipStreamSynchronize(g_stream));
+    return FL_OK;
+}
+
+// fl_run_steps without any allocation: the fused variant's steps launched one by one, the
+// run control block and the source samples in the device's scratch array.  Calls that
+// overlap on one GPU share that array and race.
+extern "C" int fl_run_steps_static(int it0, int nsteps, int iSource, const float *src)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "fl_run_steps_static before DRIVER_Initialize");
+    if (g_variant != FL_VARIANT_FUSED) return fail(FL_ERR_ARG, "fl_run_steps_static runs the fused variant only");
+    if (nsteps <= 0) return FL_OK;
+    if (nsteps > FL_STATIC_MAX_STEPS)
+        return fail(FL_ERR_ARG, "fl_run_steps_static: %d steps exceed the scratch limit of %d per call", nsteps,
+                    FL_STATIC_MAX_STEPS);
+    if (!src) return fail(FL_ERR_ARG, "src must not be NULL");
+    if (iSource < 0 || (long long)iSource >= D.n) return fail(FL_ERR_ARG, "iSource outside the grid");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    char *const scr = (dev >= 0 && dev < FL_MAX_DEVICES) ? g_scratch_dev[dev] : nullptr;
+    if (!scr) return fail(FL_ERR_STATE, "no scratch address for device %d", dev);
+    int *const run = (int *)scr;
+    float *const tab = (float *)(scr + FL_STATIC_SRC_OFF);
+    const long long sxy = (long long)D.sx * D.sy;
+    const int siz = (int)(iSource / sxy), siy = (int)((iSource % sxy) / D.sx), six = (int)(iSource % D.sx);
+    const bool src_in_domain = siz >= D.bord && siz < D.sz - D.bord && siy >= D.bord &&
+                               siy < D.sy - D.bord && six >= D.bord && six < D.sx - D.bord;
+    const long long psrc = pitched_index(iSource, D.sx, D.sy, D.px);
+    if (!src_in_domain) {
+        // the source cell is never stored by the step kernel: insert before every step
+        for (int k = 0; k < nsteps; k++) {
+            hipLaunchKernelGGL(insert_source, dim3(1), dim3(64), 0, g_stream, D.lv[D.lc], psrc, src[k]);
+            StepArgs a = base_args();
+            a.it_host = it0 + 1 + k;
+            int rc = launch_steps(a, g_tiles.ty, D.cm, false, g_stream);
+            if (rc) return rc;
+            swap_levels();
+        }
+    } else {
+        HIPCHK(hipMemcpyAsync(tab, src, (size_t)nsteps * 4, hipMemcpyHostToDevice, g_stream));
+        hipLaunchKernelGGL(set_run, dim3(1), dim3(64), 0, g_stream, run, it0, nsteps);  // counters from 0
+        hipLaunchKernelGGL(insert_source, dim3(1), dim3(64), 0, g_stream, D.lv[D.lc], psrc, src[0]);
+        for (int k = 0; k < nsteps; k++) {
+            StepArgs a = base_args();
+            a.run = run;
+            a.par = k & 1;
+            a.src_tab = tab;
+            a.src_idx = psrc;
+            int rc = launch_steps(a, g_tiles.ty, D.cm, false, g_stream);
+            if (rc) return rc;
+            swap_levels();
+        }
     }
     HIPCHK(hipGetLastError());
     if (g_sync) HIPCHK(hipStreamSynchronize(g_stream));

@@ -98,6 +98,14 @@ int fl_get_traces(float *host, int max_traces);
  * nsteps): only the first such call pays for capture and instantiation. */
 int fl_run_steps(int it0, int nsteps, int iSource, const float *src, int graph_chunk);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The steps
+ * of fl_run_steps (fused variant) launched one by one, allocating nothing: the run's step
+ * counters and its source samples live in a fixed scratch array of the library on each GPU,
+ * at most 65536 samples per call (FL_ERR_ARG beyond).  src is host memory.  The array keeps
+ * the last call's contents; calls that overlap on one GPU share it and race. */
+int fl_run_steps_static(int it0, int nsteps, int iSource, const float *src);
+
 /* All four fields (and the recorded traces) back to zero, levels in their initial roles,
  * device state and graphs kept: a run can be repeated from rest. */
 int fl_reset_fields(void);

@@ -77,6 +77,8 @@ def lib():
         "fl_get_traces": (i, [_f32p, i]),
         "fl_run_steps": (i, [i, i, i, _f32p, i]),
         "fl_reset_fields": (i, []),
+        "fl_initialize_streamed": (i, [i, i, i, i, f, f, f, f, vp, vp]),
+        "fl_set_device_precompute": (i, [i]),
         "fl_run_steps_static": (i, [i, i, i, _f32p]),
         "fl_synchronize": (i, []),
         "fl_set_timing": (i, [i]),
@@ -85,6 +87,8 @@ def lib():
         "fl_region_end": (i, [C.POINTER(d)]),
         "fl_device_name": (i, [C.c_char_p, i]),
         "fl_get_derivative_volume": (i, [i, _f32p]),
+        "fl_record_bytes": (ll, []),
+        "fl_get_records": (i, [vp]),
         "fl_device_bytes": (ll, []),
         "fl_device_memory": (i, [C.POINTER(ll), C.POINTER(ll)]),
         "fl_bytes_per_cell": (i, []),
@@ -118,6 +122,11 @@ def lib():
         "fl_model_layered": (i, [vp] + [i] * 8 + [_f32p] * 6),
         "fl_layered_model_defaults": (None, [vp, C.c_uint64]),
         "fl_cfl_dt": (f, [C.c_int64, _f32p, _f32p, f, f]),
+        "fl_model_stream_gradient": (vp, [vp] + [i] * 8),
+        "fl_model_stream_layered": (vp, [vp] + [i] * 8),
+        "fl_model_stream_fill": (i, [vp, i, i] + [_f32p] * 6),
+        "fl_model_stream_cfl_dt": (f, [vp, f, f]),
+        "fl_model_stream_close": (None, [vp]),
         "fl_f64_to_f16": (C.c_uint16, [d]),
         "fl_f64_to_bf16": (C.c_uint16, [d]),
         "fl_angle_u16": (C.c_uint16, [d]),
@@ -270,6 +279,43 @@ def device_memory():
     return f.value, t.value
 
 
+class ModelStream:
+    """A seeded gradient or layered model produced in z slabs (fl_model_stream_*): the planes
+    of model_gradient / model_layered without the full arrays on the host."""
+
+    def __init__(self, kind, n3, absorb, seed, bord=BORD, **overrides):
+        L = lib()
+        if kind == "gradient":
+            p = _Gradient()
+            L.fl_gradient_model_defaults(C.byref(p), seed)
+        else:
+            p = _Layered()
+            L.fl_layered_model_defaults(C.byref(p), seed)
+        for k, v in overrides.items():
+            setattr(p, k, v)
+        self.shape = grid_shape(n3[0], n3[1], n3[2], absorb, bord)
+        sz, sy, sx = self.shape
+        opener = L.fl_model_stream_gradient if kind == "gradient" else L.fl_model_stream_layered
+        self.handle = opener(C.byref(p), sx, sy, sz, n3[0], n3[1], n3[2], bord, absorb)
+        if not self.handle:
+            raise FletcherError("model stream: host allocation failed")
+
+    def fill(self, z0, nz):
+        """Planes z0 .. z0+nz-1 of the six arrays (slabs in increasing z from plane 0)."""
+        sz, sy, sx = self.shape
+        out = [np.empty((nz, sy, sx), np.float32) for _ in range(6)]
+        _check(lib().fl_model_stream_fill(self.handle, z0, nz, *out))
+        return out
+
+    def cfl_dt(self, h, courant=0.4):
+        return lib().fl_model_stream_cfl_dt(self.handle, h, courant)
+
+    def close(self):
+        if self.handle:
+            lib().fl_model_stream_close(self.handle)
+            self.handle = None
+
+
 def cfl_dt(model, h, courant=0.4):
     return lib().fl_cfl_dt(model[0].size, model[0], model[2], h, courant)
 
@@ -315,6 +361,19 @@ class Driver:
         self.L.fl_clear_error()
         self.L.DRIVER_Initialize(sx, sy, sz, bord, d[0], d[1], d[2], dt, *model, *ptrs)
         self._status()
+
+    def initialize_streamed(self, stream, d, dt, bord=BORD):
+        """DRIVER_Initialize from a ModelStream: the model goes to the device in z slabs and the
+        coefficient records are computed there; no whole model arrays on the host."""
+        sz, sy, sx = stream.shape
+        self.shape, self.bord, self.d, self.dt = tuple(stream.shape), bord, tuple(d), dt
+        fill = C.cast(self.L.fl_model_stream_fill, C.c_void_p)
+        self.L.fl_clear_error()
+        _check(self.L.fl_initialize_streamed(sx, sy, sz, bord, d[0], d[1], d[2], dt, fill, stream.handle))
+
+    def set_device_precompute(self, on):
+        """DRIVER_Initialize computes the coefficient records on the device (1) or the host (0)."""
+        _check(self.L.fl_set_device_precompute(1 if on else 0))
 
     def insert_source(self, it, isource, src):
         self.L.DRIVER_InsertSource(self.dt, it, isource, None, None, src)
@@ -370,6 +429,12 @@ class Driver:
         for it in range(it0 + 1, it0 + st + 1):
             self.insert_source(it - 1, isource, source(self.dt, it - 1))
             self.propagate(it)
+
+    def records(self):
+        """The device coefficient records (raw bytes, one record per pitched cell)."""
+        out = np.empty(_check(self.L.fl_record_bytes()), np.uint8)
+        _check(self.L.fl_get_records(out.ctypes.data))
+        return out
 
     def derivative_volume(self, k):
         """Two-pass first-derivative volume k (0 pDx, 1 pDy, 2 qDx, 3 qDy) of the last step."""

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "../host/fl_cell.h"
 #include "fl_launch.h"
 #include "fletcher/driver.h"
 #include "fletcher/fl_hip.h"
@@ -226,6 +227,175 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
         }
     }
     return FL_OK;
+}
+
+// ------------------------------------------------------------------ device precompute
+// The coefficient records computed on the device in FP64 from the six FP32 model volumes
+// (pitched like the levels), with the per-cell code of the host path (fl_cell.h): the same
+// records wherever the device's double sine and cosine equal the host library's.
+struct ModelVols {
+    const float *m[6];  // vpz, vsv, epsilon, delta, phi, theta
+};
+
+// Largest velocity factor need of the grid (the block floating-point exponent base).
+__global__ __launch_bounds__(256) void q16_need_max(ModelVols v, long long np, int px, int sx, double dt2,
+                                                    unsigned long long *out)
+{
+    __shared__ double red[256];
+    double m = 0.0;
+    for (long long c = blockIdx.x * 256ll + threadIdx.x; c < np; c += (long long)gridDim.x * 256) {
+        if ((int)(c % px) >= sx) continue;
+        const double need = fl_cell_q16_need(v.m[0][c], v.m[1][c], v.m[2][c], v.m[3][c], dt2);
+        if (need > m) m = need;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s && red[threadIdx.x + s] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
+        __syncthreads();
+    }
+    // non-negative doubles order as their bit patterns
+    if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// Records of every pitched cell (zero in the row padding); *bad receives 1 + the index of a
+// cell whose FP16 velocity factor overflows.
+template <int CM>
+__global__ __launch_bounds__(256) void precompute_records(ModelVols v, long long np, int px, int sx, double dt,
+                                                          int ebase, void *rec, unsigned long long *bad)
+{
+    for (long long c = blockIdx.x * 256ll + threadIdx.x; c < np; c += (long long)gridDim.x * 256) {
+        const bool pad = (int)(c % px) >= sx;
+        const float vp = pad ? 0.f : v.m[0][c], vs = pad ? 0.f : v.m[1][c], e = pad ? 0.f : v.m[2][c];
+        const float d = pad ? 0.f : v.m[3][c], ph = pad ? 0.f : v.m[4][c], th = pad ? 0.f : v.m[5][c];
+        if constexpr (CM == CM_F32) {
+            float k[10];
+            fl_cell_f32(vp, vs, e, d, ph, th, k);
+            float *r = (float *)rec + c * 10;
+            for (int s = 0; s < 10; s++) r[s] = pad ? 0.f : k[rec_order32(s)];
+        } else if constexpr (CM == CM_F32R) {
+            float k[6];
+            fl_cell_recompute32(vp, vs, e, d, ph, th, k);
+            float *r = (float *)rec + c * 6;
+            for (int s = 0; s < 6; s++) r[s] = pad ? 0.f : k[rec_order_r32(s)];
+        } else if constexpr (CM == CM_BF16 || CM == CM_F16) {
+            uint16_t k[10];
+            fl_cell_pack16(vp, vs, e, d, ph, th, dt, CM == CM_BF16, k);
+            uint16_t *r = (uint16_t *)rec + c * 10;
+            for (int s = 0; s < 10; s++) r[s] = pad ? 0 : k[rec_order16(s)];
+            if (CM == CM_F16 && !pad)
+                for (int s = 6; s < 10; s++)
+                    if ((k[s] & 0x7c00u) == 0x7c00u) atomicMax(bad, (unsigned long long)c + 1);
+        } else {
+            uint16_t k[6];
+            if constexpr (CM == CM_F16R) fl_cell_recompute16(vp, vs, e, d, ph, th, dt, k);
+            else fl_cell_recompute_q16(vp, vs, e, d, ph, th, dt, ebase, k);
+            uint16_t *r = (uint16_t *)rec + c * 6;
+            for (int s = 0; s < 6; s++) r[s] = pad ? 0 : k[rec_order_r16(s)];
+            if (CM == CM_F16R && !pad)
+                for (int s = 0; s < 4; s++)
+                    if ((k[s] & 0x7c00u) == 0x7c00u) atomicMax(bad, (unsigned long long)c + 1);
+        }
+    }
+}
+
+// Where DRIVER_Initialize and fl_initialize_streamed take the model from: six whole host
+// arrays (reference layout), or slabs from a callback.
+struct ModelSrc {
+    const float *a[6];
+    fl_model_slab_fn fill;
+    void *user;
+    int sx, sy;
+};
+
+static int model_slab(const ModelSrc &src, int z0, int nzs, float *const *out)
+{
+    if (src.fill) return src.fill(src.user, z0, nzs, out[0], out[1], out[2], out[3], out[4], out[5]);
+    const size_t plane = (size_t)src.sx * src.sy;
+    for (int k = 0; k < 6; k++) memcpy(out[k], src.a[k] + (size_t)z0 * plane, plane * nzs * 4);
+    return 0;
+}
+
+static bool g_device_precompute = false;
+
+extern "C" int fl_set_device_precompute(int on)
+{
+    g_device_precompute = on != 0;
+    return FL_OK;
+}
+
+// The model streamed to the device in z slabs (pinned staging of at most 6 x 64 MiB), the
+// records of form cm computed there into dst; *vq receives 2^ebase for CM_Q16R.
+static int device_records(const ModelSrc &src, int cm, int sx, int sy, int sz, int px, float dt, void *dst, float *vq)
+{
+    const long long np = (long long)px * sy * sz;
+    const long long plane = (long long)sx * sy;
+    const int slab = (int)std::max<long long>(1, std::min<long long>(sz, (64ll << 20) / (plane * 4)));
+    float *dv[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    float *stage = nullptr;
+    unsigned long long *scal = nullptr;  // [0] q16 need, [1] overflow cell + 1
+    long long dummy = 0;
+    int rc = FL_OK;
+    for (int k = 0; k < 6 && rc == FL_OK; k++) rc = dev_alloc((void **)&dv[k], (size_t)np * 4, &dummy);
+    if (rc == FL_OK) rc = dev_alloc((void **)&scal, 2 * sizeof(unsigned long long), &dummy);
+    if (rc == FL_OK && hipHostMalloc((void **)&stage, (size_t)6 * slab * plane * 4, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        stage = nullptr;
+        rc = fail(FL_ERR_NOMEM, "pinned staging of %lld bytes failed", 6ll * slab * plane * 4);
+    }
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 6 && rc == FL_OK && e == hipSuccess; k++) e = hipMemsetAsync(dv[k], 0, (size_t)np * 4, g_stream);
+    if (rc == FL_OK && e == hipSuccess) e = hipMemsetAsync(scal, 0, 2 * sizeof(unsigned long long), g_stream);
+    float *h[6];
+    for (int k = 0; k < 6; k++) h[k] = stage + (size_t)k * slab * plane;
+    for (int z0 = 0; z0 < sz && rc == FL_OK && e == hipSuccess; z0 += slab) {
+        const int n = std::min(slab, sz - z0);
+        e = hipStreamSynchronize(g_stream);  // the staging buffer is free again
+        if (e != hipSuccess) break;
+        if (model_slab(src, z0, n, h)) {
+            rc = fail(FL_ERR_ARG, "the model source failed for planes %d..%d", z0, z0 + n - 1);
+            break;
+        }
+        for (int k = 0; k < 6 && e == hipSuccess; k++)
+            e = hipMemcpy2DAsync(dv[k] + (size_t)z0 * sy * px, (size_t)px * 4, h[k], (size_t)sx * 4, (size_t)sx * 4,
+                                 (size_t)sy * n, hipMemcpyHostToDevice, g_stream);
+    }
+    ModelVols mv;
+    for (int k = 0; k < 6; k++) mv.m[k] = dv[k];
+    const int grid = (int)std::min<long long>((np + 255) / 256, 65536);
+    int ebase = 0;
+    if (rc == FL_OK && e == hipSuccess && cm == CM_Q16R) {
+        hipLaunchKernelGGL(q16_need_max, dim3(grid), dim3(256), 0, g_stream, mv, np, px, sx, (double)dt * dt, scal);
+        unsigned long long bits = 0;
+        e = hipMemcpyAsync(&bits, scal, 8, hipMemcpyDeviceToHost, g_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+        double m;
+        memcpy(&m, &bits, 8);
+        ebase = fl_cell_q16_base_of(m);
+        *vq = std::ldexp(1.0f, ebase);
+    }
+    if (rc == FL_OK && e == hipSuccess) {
+        switch (cm) {
+#define FL_PRE(CMV) \
+    case CMV: hipLaunchKernelGGL(precompute_records<CMV>, dim3(grid), dim3(256), 0, g_stream, mv, np, px, sx, (double)dt, ebase, dst, scal + 1); break;
+            FL_PRE(CM_F32) FL_PRE(CM_BF16) FL_PRE(CM_F16) FL_PRE(CM_F16R) FL_PRE(CM_F32R) FL_PRE(CM_Q16R)
+#undef FL_PRE
+        }
+        unsigned long long badc = 0;
+        e = hipMemcpyAsync(&badc, scal + 1, 8, hipMemcpyDeviceToHost, g_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+        if (e == hipSuccess && badc)
+            rc = fail(FL_ERR_ARG,
+                      "velocity factor v^2 dt^2 of pitched cell %llu exceeds the FP16 range (v dt > 256 m): "
+                      "use FP32, BF16 or block floating-point storage", badc - 1);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    for (int k = 0; k < 6; k++)
+        if (dv[k]) (void)hipFree(dv[k]);
+    if (scal) (void)hipFree(scal);
+    if (stage) (void)hipHostFree(stage);
+    if (rc == FL_OK && e != hipSuccess) rc = fail(FL_ERR_HIP, "device precompute: %s", hipGetErrorString(e));
+    return rc;
 }
 
 // ------------------------------------------------------------------ launch plumbing
@@ -514,17 +684,18 @@ static void free_device()
     D = Device();
 }
 
-static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, float dz, float dt,
-                      const float *vpz, const float *vsv, const float *eps, const float *del,
-                      const float *phi, const float *theta, const float *pp, const float *pc,
-                      const float *qp, const float *qc)
+// Device state for a grid: levels (pp, qp, pc, qc from host arrays or zero), coefficient
+// records from the model source, computed on the host (whole arrays, unless device
+// precompute is selected) or on the device (always for slab sources).
+static int initialize_src(int sx, int sy, int sz, int bord, float dx, float dy, float dz, float dt,
+                          const ModelSrc &src, const float *pp, const float *pc, const float *qp, const float *qc)
 {
     if (bord < 4) return fail(FL_ERR_ARG, "bord=%d: the 8th-order stencil needs bord >= 4", bord);
     if (sx <= 2 * bord || sy <= 2 * bord || sz <= 2 * bord)
         return fail(FL_ERR_ARG, "grid %dx%dx%d has no interior for bord=%d", sx, sy, sz, bord);
     if (!(dx > 0 && dy > 0 && dz > 0 && dt > 0))
         return fail(FL_ERR_ARG, "spacings and dt must be positive");
-    if (!vpz || !vsv || !eps || !del || !phi || !theta)
+    if (!src.fill && (!src.a[0] || !src.a[1] || !src.a[2] || !src.a[3] || !src.a[4] || !src.a[5]))
         return fail(FL_ERR_ARG, "model arrays must not be NULL");
     if (max_zt(pitch_of(sx), sy) < 1)
         return fail(FL_ERR_ARG, "xy plane %dx%d too large for 32-bit plane offsets", sx, sy);
@@ -557,12 +728,18 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
     }
     const size_t cbytes = (size_t)coef_count(D.cm) * D.np * coef_elem(D.cm);
     if ((rc = dev_alloc(&D.coef, cbytes, &D.bytes))) { free_device(); return rc; }
-    int ebase = 0;
-    if (D.cm == CM_Q16R) {
-        ebase = fl_recompute_q16_base(D.n, vpz, vsv, eps, del, dt);
-        D.vq = std::ldexp(1.0f, ebase);
+    if (src.fill || g_device_precompute) {
+        rc = device_records(src, D.cm, sx, sy, sz, D.px, dt, D.coef, &D.vq);
+    } else {
+        int ebase = 0;
+        if (D.cm == CM_Q16R) {
+            ebase = fl_recompute_q16_base(D.n, src.a[0], src.a[1], src.a[2], src.a[3], dt);
+            D.vq = std::ldexp(1.0f, ebase);
+        }
+        rc = upload_coefs(src.a[0], src.a[1], src.a[2], src.a[3], src.a[4], src.a[5], dt, D.cm, D.coef, sx, sy, sz,
+                          D.px, ebase);
     }
-    if ((rc = upload_coefs(vpz, vsv, eps, del, phi, theta, dt, D.cm, D.coef, sx, sy, sz, D.px, ebase))) {
+    if (rc) {
         free_device();
         return rc;
     }
@@ -571,9 +748,39 @@ static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, floa
     return FL_OK;
 }
 
+static int initialize(int sx, int sy, int sz, int bord, float dx, float dy, float dz, float dt,
+                      const float *vpz, const float *vsv, const float *eps, const float *del,
+                      const float *phi, const float *theta, const float *pp, const float *pc,
+                      const float *qp, const float *qc)
+{
+    ModelSrc src;
+    memset(&src, 0, sizeof(src));
+    const float *a[6] = {vpz, vsv, eps, del, phi, theta};
+    for (int k = 0; k < 6; k++) src.a[k] = a[k];
+    src.sx = sx;
+    src.sy = sy;
+    return initialize_src(sx, sy, sz, bord, dx, dy, dz, dt, src, pp, pc, qp, qc);
+}
+
+extern "C" int fl_initialize_streamed(int sx, int sy, int sz, int bord, float dx, float dy, float dz, float dt,
+                                      fl_model_slab_fn fill, void *user)
+{
+    if (!fill) return fail(FL_ERR_ARG, "fl_initialize_streamed needs a model source");
+    ModelSrc src;
+    memset(&src, 0, sizeof(src));
+    src.fill = fill;
+    src.user = user;
+    src.sx = sx;
+    src.sy = sy;
+    int rc = initialize_src(sx, sy, sz, bord, dx, dy, dz, dt, src, nullptr, nullptr, nullptr, nullptr);
+    if (rc == FL_OK) fl_clear_error();
+    return rc;
+}
+
 // Selection through the environment for front-ends that only call driver.h (the
 // reference's main.c/model.c): FL_VARIANT = fused|direct|twopass, FL_PRECISION =
-// fp32|bf16|fp16|fp16r|fp32r|q16r, FL_STREAM_HINTS = 0|1, applied at DRIVER_Initialize when set;
+// fp32|bf16|fp16|fp16r|fp32r|q16r, FL_STREAM_HINTS = 0|1, FL_DEVICE_PRECOMPUTE = 0|1, applied at
+// DRIVER_Initialize when set;
 // FL_TUNE = cache file, the autotuner runs (or reads the cache) right after it.
 static int env_pick(const char *name, const char *const *opts, int n)
 {
@@ -596,14 +803,15 @@ extern "C" void DRIVER_Initialize(const int sx, const int sy, const int sz, cons
     static const char *const precs[] = {"fp32", "bf16", "fp16", "fp16r", "fp32r", "q16r"};
     static const char *const onoff[] = {"0", "1"};
     const int v = env_pick("FL_VARIANT", variants, 3), p = env_pick("FL_PRECISION", precs, 6);
-    const int h = env_pick("FL_STREAM_HINTS", onoff, 2);
-    if (v == -2 || p == -2 || h == -2) {
-        fail(FL_ERR_ARG, "FL_VARIANT / FL_PRECISION / FL_STREAM_HINTS hold an unknown value");
+    const int h = env_pick("FL_STREAM_HINTS", onoff, 2), dp = env_pick("FL_DEVICE_PRECOMPUTE", onoff, 2);
+    if (v == -2 || p == -2 || h == -2 || dp == -2) {
+        fail(FL_ERR_ARG, "FL_VARIANT / FL_PRECISION / FL_STREAM_HINTS / FL_DEVICE_PRECOMPUTE hold an unknown value");
         return;
     }
     if (v >= 0) g_variant = v;
     if (p >= 0) g_precision = p;
     if (h >= 0) g_tiles.hints = h;
+    if (dp >= 0) g_device_precompute = dp != 0;
     if (initialize(sx, sy, sz, bord, dx, dy, dz, dt, vpz, vsv, epsilon, delta, phi, theta, pp, pc, qp, qc) != FL_OK)
         return;
     fl_clear_error();
@@ -741,6 +949,21 @@ extern "C" int fl_set_field(int which, const float *host)
     if (which < 0 || which > 3 || !host) return fail(FL_ERR_ARG, "bad field request %d", which);
     float *lv = (which & 1) ? D.lv[D.lc] : D.lv[1 - D.lc];
     return level_xfer(lv, which >> 1, const_cast<float *>(host), true, D.sx, D.sy, D.sz, D.px);
+}
+
+extern "C" long long fl_record_bytes(void)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "no initialised grid");
+    return (long long)coef_count(D.cm) * coef_elem(D.cm) * D.np;
+}
+
+extern "C" int fl_get_records(void *host)
+{
+    if (!D.init) return fail(FL_ERR_STATE, "no initialised grid");
+    if (!host) return fail(FL_ERR_ARG, "host buffer must not be NULL");
+    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipMemcpy(host, D.coef, (size_t)fl_record_bytes(), hipMemcpyDeviceToHost));
+    return FL_OK;
 }
 
 extern "C" int fl_get_derivative_volume(int k, float *host)

@@ -8,6 +8,8 @@
 #include "fletcher/fl_model.h"
 
 #include <math.h>
+
+#include "fl_cell.h"
 #include <stdlib.h>
 #include <string.h>
 
@@ -99,36 +101,33 @@ int fl_num_steps(float tmax, float dt)
     return (int)ceil(tmax / dt); /* main.c:101, float quotient */
 }
 
-void fl_random_boundary(uint32_t seed, int sx, int sy, int sz, int nx, int ny, int nz, int bord,
-                        int absorb, float *vpz, float *vsv)
+void fl_boundary_planes(fl_rand_t *rs, int sx, int sy, int nx, int ny, int nz, int bord, int absorb,
+                        float maxP, float maxS, const float *elo_vpz, const float *elo_vsv,
+                        const float *ehi_vpz, const float *ehi_vsv, int z0, int nzs, float *vpz, float *vsv)
 {
-    /* boundary.c:7-98 */
-#define IND(ix, iy, iz) ((((int64_t)(iz)) * sy + (iy)) * (int64_t)sx + (ix))
-    fl_rand_t rs;
-    fl_rand_seed(&rs, seed);
-    float maxP = 0.0, maxS = 0.0;
-    for (int iz = bord + absorb; iz < nz + bord + absorb; iz++)
-        for (int iy = bord + absorb; iy < ny + bord + absorb; iy++)
-            for (int64_t i = IND(bord + absorb, iy, iz); i < IND(nx + bord + absorb, iy, iz); i++) {
-                maxP = fmaxf(vpz[i], maxP);
-                maxS = fmaxf(vsv[i], maxS);
-            }
+    /* boundary.c:26-98 for planes z0 .. z0+nzs-1, in the reference's iz -> iy -> ix order */
+#define PIX(ix, iy) ((int64_t)(iy) * sx + (ix))
     const int bordLen = bord + absorb - 1; /* last index of the low absorbing zone */
     const int firstIn = bordLen + 1;       /* first interior index */
     const float frac = 1.0 / (float)(absorb);
-    for (int iz = 0; iz < sz; iz++)
+    const int64_t plane = (int64_t)sx * sy;
+    for (int iz = z0; iz < z0 + nzs; iz++) {
+        float *pv = vpz + (iz - z0) * plane, *ps = vsv + (iz - z0) * plane;
+        /* the plane that holds the interior cell j of this plane's absorbing cells */
+        const float *jv = iz < firstIn ? elo_vpz : iz > bordLen + nz ? ehi_vpz : pv;
+        const float *js = iz < firstIn ? elo_vsv : iz > bordLen + nz ? ehi_vsv : ps;
         for (int iy = 0; iy < sy; iy++)
             for (int ix = 0; ix < sx; ix++) {
-                const int64_t i = IND(ix, iy, iz);
+                const int64_t i = PIX(ix, iy);
                 if ((iz >= firstIn && iz <= bordLen + nz) && (iy >= firstIn && iy <= bordLen + ny) &&
                     (ix >= firstIn && ix <= bordLen + nx))
                     continue; /* interior untouched */
                 if ((iz >= bord && iz <= 2 * bordLen + nz) && (iy >= bord && iy <= 2 * bordLen + ny) &&
                     (ix >= bord && ix <= 2 * bordLen + nx)) {
-                    int distz, disty, distx, ivelz, ively, ivelx;
-                    if (iz > bordLen + nz) { distz = iz - bordLen - nz; ivelz = bordLen + nz; }
-                    else if (iz < firstIn) { distz = firstIn - iz; ivelz = firstIn; }
-                    else { distz = 0; ivelz = iz; }
+                    int distz, disty, distx, ively, ivelx;
+                    if (iz > bordLen + nz) distz = iz - bordLen - nz;
+                    else if (iz < firstIn) distz = firstIn - iz;
+                    else distz = 0;
                     if (iy > bordLen + ny) { disty = iy - bordLen - ny; ively = bordLen + ny; }
                     else if (iy < firstIn) { disty = firstIn - iy; ively = firstIn; }
                     else { disty = 0; ively = iy; }
@@ -138,16 +137,47 @@ void fl_random_boundary(uint32_t seed, int sx, int sy, int sz, int nx, int ny, i
                     int dist = (disty > distz) ? disty : distz;
                     dist = (dist > distx) ? dist : distx;
                     const float bordDist = (float)(dist)*frac;
-                    const float rfac = (float)fl_rand_next(&rs) / (float)2147483647;
-                    const int64_t j = IND(ivelx, ively, ivelz);
-                    vpz[i] = vpz[j] * (1.0 - bordDist) + maxP * rfac * bordDist;
-                    vsv[i] = vsv[j] * (1.0 - bordDist) + maxS * rfac * bordDist;
+                    const float rfac = (float)fl_rand_next(rs) / (float)2147483647;
+                    const int64_t j = PIX(ivelx, ively);
+                    pv[i] = jv[j] * (1.0 - bordDist) + maxP * rfac * bordDist;
+                    ps[i] = js[j] * (1.0 - bordDist) + maxS * rfac * bordDist;
                 } else {
-                    vpz[i] = 0.0;
-                    vsv[i] = 0.0;
+                    pv[i] = 0.0;
+                    ps[i] = 0.0;
                 }
             }
-#undef IND
+    }
+#undef PIX
+}
+
+void fl_boundary_max(int sx, int sy, int nx, int ny, int bord, int absorb, int z0, int nzs,
+                     const float *vpz, const float *vsv, float *maxP, float *maxS)
+{
+    /* boundary.c:12-24: the interior maxima, over the interior cells of planes z0 .. z0+nzs-1 */
+    const int b = bord + absorb;
+    const int64_t plane = (int64_t)sx * sy;
+    for (int iz = z0; iz < z0 + nzs; iz++)
+        for (int iy = b; iy < ny + b; iy++)
+            for (int64_t i = (iz - z0) * plane + (int64_t)iy * sx + b; i < (iz - z0) * plane + (int64_t)iy * sx + nx + b;
+                 i++) {
+                *maxP = fmaxf(vpz[i], *maxP);
+                *maxS = fmaxf(vsv[i], *maxS);
+            }
+}
+
+void fl_random_boundary(uint32_t seed, int sx, int sy, int sz, int nx, int ny, int nz, int bord,
+                        int absorb, float *vpz, float *vsv)
+{
+    /* boundary.c:7-98 */
+    fl_rand_t rs;
+    fl_rand_seed(&rs, seed);
+    const int b = bord + absorb;
+    const int64_t plane = (int64_t)sx * sy;
+    float maxP = 0.0, maxS = 0.0;
+    fl_boundary_max(sx, sy, nx, ny, bord, absorb, b, nz, vpz + b * plane, vsv + b * plane, &maxP, &maxS);
+    const int64_t lo = (int64_t)b * plane, hi = (int64_t)(b + nz - 1) * plane;
+    fl_boundary_planes(&rs, sx, sy, nx, ny, nz, bord, absorb, maxP, maxS, vpz + lo, vsv + lo, vpz + hi, vsv + hi,
+                       0, sz, vpz, vsv);
 }
 
 float fl_source(float dt, int it)
@@ -166,26 +196,12 @@ float fl_source(float dt, int it)
 void fl_precompute_f32(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                        const float *delta, const float *phi, const float *theta, float *const *c)
 {
-    /* precomp.h:25-38 and :53-58 */
+    /* precomp.h:25-38 and :53-58 (fl_cell_f32) */
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
-        const float sinTheta = sin(theta[i]);
-        const float cosTheta = cos(theta[i]);
-        const float sin2Theta = sin(2.0 * theta[i]);
-        const float sinPhi = sin(phi[i]);
-        const float cosPhi = cos(phi[i]);
-        const float sin2Phi = sin(2.0 * phi[i]);
-        c[0][i] = sinTheta * sinTheta * cosPhi * cosPhi;
-        c[1][i] = sinTheta * sinTheta * sinPhi * sinPhi;
-        c[2][i] = cosTheta * cosTheta;
-        c[3][i] = sinTheta * sinTheta * sin2Phi;
-        c[4][i] = sin2Theta * sinPhi;
-        c[5][i] = sin2Theta * cosPhi;
-        const float v2pz = vpz[i] * vpz[i];
-        c[8][i] = vsv[i] * vsv[i];
-        c[7][i] = v2pz;
-        c[6][i] = v2pz * (1.0 + 2.0 * epsilon[i]);
-        c[9][i] = v2pz * (1.0 + 2.0 * delta[i]);
+        float v[10];
+        fl_cell_f32(vpz[i], vsv[i], epsilon[i], delta[i], phi[i], theta[i], v);
+        for (int k = 0; k < 10; k++) c[k][i] = v[k];
     }
 }
 
@@ -194,21 +210,9 @@ void fl_precompute_f64(int64_t n, const float *vpz, const float *vsv, const floa
 {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
-        const double st = sin((double)theta[i]), ct = cos((double)theta[i]);
-        const double s2t = sin(2.0 * theta[i]);
-        const double sp = sin((double)phi[i]), cp = cos((double)phi[i]);
-        const double s2p = sin(2.0 * phi[i]);
-        c[0][i] = st * st * cp * cp;
-        c[1][i] = st * st * sp * sp;
-        c[2][i] = ct * ct;
-        c[3][i] = st * st * s2p;
-        c[4][i] = s2t * sp;
-        c[5][i] = s2t * cp;
-        const double v2pz = (double)vpz[i] * vpz[i];
-        c[8][i] = (double)vsv[i] * vsv[i];
-        c[7][i] = v2pz;
-        c[6][i] = v2pz * (1.0 + 2.0 * epsilon[i]);
-        c[9][i] = v2pz * (1.0 + 2.0 * delta[i]);
+        double v[10];
+        fl_cell_f64(vpz[i], vsv[i], epsilon[i], delta[i], phi[i], theta[i], v);
+        for (int k = 0; k < 10; k++) c[k][i] = v[k];
     }
 }
 
@@ -234,123 +238,34 @@ void fl_extract_window(const float *src, int sx, int sy, int sz, int x0, int y0,
 
 /* ------------------------------------------------------------------ 16-bit packs */
 
-uint16_t fl_f64_to_f16(double v)
-{
-    const uint16_t sign = signbit(v) ? 0x8000 : 0;
-    if (isnan(v)) return 0x7e00;
-    const double a = fabs(v);
-    if (a >= 65520.0) return sign | 0x7c00; /* rounds to infinity */
-    int e;
-    const double m = frexp(a, &e); /* a = m 2^e, 0.5 <= m < 1 */
-    if (a != 0.0 && e - 1 >= -14) {
-        double q = nearbyint(ldexp(m, 11)); /* 11-bit significand, ties to even */
-        int E = e - 1;
-        if (q >= 2048.0) {
-            q = 1024.0;
-            E += 1;
-        }
-        return (uint16_t)(sign | ((E + 15) << 10) | ((int)q - 1024));
-    }
-    /* subnormal range: multiples of 2^-24 (1024 * 2^-24 encodes the smallest normal) */
-    return (uint16_t)(sign | (uint16_t)nearbyint(ldexp(a, 24)));
-}
+uint16_t fl_f64_to_f16(double v) { return fl_cell_to_f16(v); }
 
-uint16_t fl_f64_to_bf16(double v)
-{
-    if (isnan(v)) return 0x7fc0;
-    double r = v;
-    if (v != 0.0 && isfinite(v)) {
-        int e;
-        const double m = frexp(v, &e);
-        r = ldexp(nearbyint(ldexp(m, 8)), e - 8); /* 8 significant bits, ties to even */
-    }
-    const float f = (float)r; /* exact within the float range */
-    uint32_t u;
-    memcpy(&u, &f, 4);
-    return (uint16_t)(u >> 16);
-}
+uint16_t fl_f64_to_bf16(double v) { return fl_cell_to_bf16(v); }
+
+uint16_t fl_angle_u16(double turns) { return fl_cell_angle_u16(turns); }
 
 void fl_pack_coefs16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                      const float *delta, const float *phi, const float *theta, double dt, int bf16,
                      uint16_t *const *out)
 {
-    const double dt2 = dt * dt;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
-        const double st = sin((double)theta[i]), ct = cos((double)theta[i]);
-        const double s2t = sin(2.0 * theta[i]);
-        const double sp = sin((double)phi[i]), cp = cos((double)phi[i]);
-        const double s2p = sin(2.0 * phi[i]);
-        const double vp2 = (double)vpz[i] * vpz[i];
-        const double c[10] = {st * st * cp * cp, st * st * sp * sp, ct * ct, st * st * s2p,
-                              s2t * sp, s2t * cp, vp2 * (1.0 + 2.0 * epsilon[i]) * dt2, vp2 * dt2,
-                              (double)vsv[i] * vsv[i] * dt2, vp2 * (1.0 + 2.0 * delta[i]) * dt2};
-        for (int k = 0; k < 10; k++) out[k][i] = bf16 ? fl_f64_to_bf16(c[k]) : fl_f64_to_f16(c[k]);
+        uint16_t v[10];
+        fl_cell_pack16(vpz[i], vsv[i], epsilon[i], delta[i], phi[i], theta[i], dt, bf16, v);
+        for (int k = 0; k < 10; k++) out[k][i] = v[k];
     }
-}
-
-uint16_t fl_angle_u16(double turns)
-{
-    const double t = turns - floor(turns);           /* [0, 1) */
-    return (uint16_t)((uint32_t)nearbyint(t * 65536.0) & 0xffffu);
 }
 
 void fl_pack_recompute16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, double dt,
                          uint16_t *const *out)
 {
-    const double pi = 3.14159265358979323846, dt2 = dt * dt;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
-        /* the four velocity factors of fl_precompute_f64, times dt^2 */
-        const double v2pz = (double)vpz[i] * vpz[i];
-        out[0][i] = fl_f64_to_f16(v2pz * (1.0 + 2.0 * epsilon[i]) * dt2);
-        out[1][i] = fl_f64_to_f16(v2pz * dt2);
-        out[2][i] = fl_f64_to_f16((double)vsv[i] * vsv[i] * dt2);
-        out[3][i] = fl_f64_to_f16(v2pz * (1.0 + 2.0 * delta[i]) * dt2);
-        /* angles as 16-bit fixed point in revolutions, round to nearest, modulo one turn */
-        out[4][i] = fl_angle_u16((double)theta[i] / (2.0 * pi));
-        out[5][i] = fl_angle_u16((double)phi[i] / (2.0 * pi));
+        uint16_t v[6];
+        fl_cell_recompute16(vpz[i], vsv[i], epsilon[i], delta[i], phi[i], theta[i], dt, v);
+        for (int k = 0; k < 6; k++) out[k][i] = v[k];
     }
-}
-
-/* Block floating point of the four FP64 velocity factors times dt^2 of one cell: 15-bit
- * mantissas q * 2^E of one exponent E shared by the cell, the smallest that holds the four,
- * clamped to [emin, emin + 15]. */
-static int q16_block(const double f[4], int emin, uint32_t q[4])
-{
-    double r = 0.0;
-    for (int k = 0; k < 4; k++) r = f[k] / 32767.5 > r ? f[k] / 32767.5 : r;
-    int e = emin;
-    if (r > 0.0) {
-        int ex;
-        frexp(r, &ex);  /* r = fr * 2^ex, fr in [0.5, 1): 2^ex is the first power above r */
-        e = ex;
-        while (e > emin) {  /* the smallest E whose rounded mantissas still fit */
-            int fits = 1;
-            for (int k = 0; k < 4; k++)
-                if (nearbyint(ldexp(f[k], -(e - 1))) > 32767.0) fits = 0;
-            if (!fits) break;
-            e--;
-        }
-        if (e < emin) e = emin;
-        if (e > emin + 15) e = emin + 15;
-    }
-    for (int k = 0; k < 4; k++) {
-        const double u = nearbyint(ldexp(f[k], -e));
-        q[k] = (uint32_t)(u < 0.0 ? 0.0 : (u > 32767.0 ? 32767.0 : u));
-    }
-    return e - emin;
-}
-
-static void q16_factors(int64_t i, const float *vpz, const float *vsv, const float *epsilon,
-                        const float *delta, double dt2, double f[4])
-{
-    const double v2pz = (double)vpz[i] * vpz[i];
-    f[0] = v2pz * (1.0 + 2.0 * epsilon[i]) * dt2;
-    f[1] = v2pz * dt2;
-    f[2] = (double)vsv[i] * vsv[i] * dt2;
-    f[3] = v2pz * (1.0 + 2.0 * delta[i]) * dt2;
 }
 
 int fl_recompute_q16_base(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
@@ -361,57 +276,32 @@ int fl_recompute_q16_base(int64_t n, const float *vpz, const float *vsv, const f
     const double dt2 = dt * dt;
 #pragma omp parallel for schedule(static) reduction(max : m)
     for (int64_t i = 0; i < n; i++) {
-        double f[4];
-        q16_factors(i, vpz, vsv, epsilon, delta, dt2, f);
-        for (int k = 0; k < 4; k++)
-            if (f[k] / 32767.5 > m) m = f[k] / 32767.5;
+        const double c = fl_cell_q16_need(vpz[i], vsv[i], epsilon[i], delta[i], dt2);
+        if (c > m) m = c;
     }
-    if (!(m > 0.0)) return 0;
-    int ex;
-    frexp(m, &ex);
-    return ex - 15;
+    return fl_cell_q16_base_of(m);
 }
 
 void fl_pack_recompute_q16(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                            const float *delta, const float *phi, const float *theta, double dt,
                            int ebase, uint16_t *const *out)
 {
-    const double pi = 3.14159265358979323846, dt2 = dt * dt;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
-        double f[4];
-        uint32_t q[4];
-        q16_factors(i, vpz, vsv, epsilon, delta, dt2, f);
-        const uint32_t e = (uint32_t)q16_block(f, ebase, q);
-        /* record words (v2px, v2pn) and (v2pz, v2sz): mantissas in bits 0-14 and 15-29,
-         * e bits 0-1 in bits 30-31 of the first word, e bits 2-3 in those of the second */
-        const uint32_t w0 = q[0] | (q[3] << 15) | ((e & 3u) << 30);
-        const uint32_t w1 = q[1] | (q[2] << 15) | ((e >> 2) << 30);
-        out[0][i] = (uint16_t)(w0 & 0xffffu);
-        out[3][i] = (uint16_t)(w0 >> 16);
-        out[1][i] = (uint16_t)(w1 & 0xffffu);
-        out[2][i] = (uint16_t)(w1 >> 16);
-        out[4][i] = fl_angle_u16((double)theta[i] / (2.0 * pi));
-        out[5][i] = fl_angle_u16((double)phi[i] / (2.0 * pi));
+        uint16_t v[6];
+        fl_cell_recompute_q16(vpz[i], vsv[i], epsilon[i], delta[i], phi[i], theta[i], dt, ebase, v);
+        for (int k = 0; k < 6; k++) out[k][i] = v[k];
     }
 }
 
 void fl_pack_recompute32(int64_t n, const float *vpz, const float *vsv, const float *epsilon,
                          const float *delta, const float *phi, const float *theta, float *const *out)
 {
-    const double pi = 3.14159265358979323846;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
-        /* the FP32 velocity factors of precomp.h:53-58, as fl_precompute_f32 forms them */
-        const float v2pz = vpz[i] * vpz[i];
-        out[0][i] = v2pz * (1.0 + 2.0 * epsilon[i]);
-        out[1][i] = v2pz;
-        out[2][i] = vsv[i] * vsv[i];
-        out[3][i] = v2pz * (1.0 + 2.0 * delta[i]);
-        /* angles in turns, reduced to [-1/2, 1/2] in FP64 and rounded once */
-        const double tt = (double)theta[i] / (2.0 * pi), tp = (double)phi[i] / (2.0 * pi);
-        out[4][i] = (float)(tt - nearbyint(tt));
-        out[5][i] = (float)(tp - nearbyint(tp));
+        float v[6];
+        fl_cell_recompute32(vpz[i], vsv[i], epsilon[i], delta[i], phi[i], theta[i], v);
+        for (int k = 0; k < 6; k++) out[k][i] = v[k];
     }
 }
 

@@ -54,8 +54,8 @@ void fl_clear_error(void);
 /* Front-ends that call only driver.h select with the environment instead, read by
  * DRIVER_Initialize: FL_VARIANT=fused|direct|twopass,
  * FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r|q16r,
- * FL_STREAM_HINTS=0|1 (fl_set_stream_hints), FL_TUNE=<cache file> (autotune right after
- * initialisation). */
+ * FL_STREAM_HINTS=0|1 (fl_set_stream_hints), FL_DEVICE_PRECOMPUTE=0|1 (fl_set_device_precompute),
+ * FL_TUNE=<cache file> (autotune right after initialisation). */
 int fl_set_variant(int variant);   /* takes effect at the next step */
 int fl_get_variant(void);
 int fl_set_precision(int precision); /* used by the next DRIVER_Initialize */
@@ -80,6 +80,25 @@ int fl_get_stream_hints(void);
  * afterwards.  When cache_path is not NULL the choice is looked up / appended there, keyed
  * by device name, grid, variant and precision. */
 int fl_autotune(int steps, const char *cache_path);
+
+/* Model source of fl_initialize_streamed: writes planes z0 .. z0+nzs-1 of the six model
+ * arrays (slab arrays of sx*sy*nzs floats, reference layout) and returns 0, non-zero to
+ * abort.  Called with z0 = 0, then each slab after the previous one. */
+typedef int (*fl_model_slab_fn)(void *user, int z0, int nzs, float *vpz, float *vsv, float *epsilon,
+                                float *delta, float *phi, float *theta);
+
+/* DRIVER_Initialize for models too large for whole host arrays (no reference counterpart:
+ * model setup of precomp.h:14-69 on the device).  The model arrives in z slabs of about
+ * 64 MiB per array through pinned staging; the coefficient records of the selected precision
+ * are computed on the device in FP64 with the per-cell code of the host path.  Fields start
+ * at zero.  fl_model_stream_fill (fl_model.h) is such a source for the seeded models. */
+int fl_initialize_streamed(int sx, int sy, int sz, int bord, float dx, float dy, float dz, float dt,
+                           fl_model_slab_fn fill, void *user);
+
+/* 1: DRIVER_Initialize computes the coefficient records on the device as well (FP64 trig
+ * of the device library; records equal to the host's wherever the two libraries' sine and
+ * cosine agree).  0 (default): host precompute, bit for bit precomp.h. */
+int fl_set_device_precompute(int on);
 
 /* Device fields after the last step: which = 0 pp, 1 pc, 2 qp, 3 qc (n floats). */
 int fl_get_field(int which, float *host);
@@ -124,6 +143,10 @@ int fl_region_end(double *ms);
 /* Diagnostics: the two-pass variant's first-derivative volume k (0 pDx, 1 pDy, 2 qDx,
  * 3 qDy; der1der1/OpenMP/openmp_propagate.c:24-76) of the last step, n floats. */
 int fl_get_derivative_volume(int k, float *host);
+/* The device coefficient records of the initialised grid: one record per pitched cell
+ * (row pitch sx rounded up to 16 cells), fl_record_bytes() bytes in all. */
+long long fl_record_bytes(void);
+int fl_get_records(void *host);
 
 int fl_device_name(char *buf, int len);
 long long fl_device_bytes(void);   /* device memory held by the back-end */

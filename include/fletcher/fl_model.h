@@ -42,6 +42,19 @@ float fl_recommended_dt(int64_t n, const float *vpz, const float *epsilon, float
 void fl_random_boundary(uint32_t seed, int sx, int sy, int sz, int nx, int ny, int nz, int bord,
                         int absorb, float *vpz, float *vsv);
 
+/* The same boundary plane by plane, for models produced in z slabs:
+ * fl_boundary_max accumulates the interior maxima (boundary.c:12-24) over the interior
+ * planes z0 .. z0+nzs-1 held in vpz, vsv (slab arrays of nzs planes; start with 0);
+ * fl_boundary_planes applies boundary.c:26-98 to the slab of planes z0 .. z0+nzs-1,
+ * continuing the rand() stream rs (seeded with fl_rand_seed, slabs in increasing z).
+ * elo_*, ehi_* are the unmodified planes iz = bord+absorb and bord+absorb+nz-1, from which
+ * the cells of the low and high z absorbing zones take their edge values. */
+void fl_boundary_max(int sx, int sy, int nx, int ny, int bord, int absorb, int z0, int nzs,
+                     const float *vpz, const float *vsv, float *maxP, float *maxS);
+void fl_boundary_planes(fl_rand_t *rs, int sx, int sy, int nx, int ny, int nz, int bord, int absorb,
+                        float maxP, float maxS, const float *elo_vpz, const float *elo_vsv,
+                        const float *ehi_vpz, const float *ehi_vsv, int z0, int nzs, float *vpz, float *vsv);
+
 /* Ricker source sample at time it*dt — source.c:7-15, source.h:11-14. */
 float fl_source(float dt, int it);
 
@@ -138,6 +151,25 @@ void fl_layered_model_defaults(fl_layered_model_t *m, uint64_t seed);
 int fl_model_layered(const fl_layered_model_t *m, int sx, int sy, int sz, int nx, int ny, int nz,
                      int bord, int absorb, float *vpz, float *vsv, float *epsilon, float *delta,
                      float *phi, float *theta);
+
+/* The same models produced in z slabs, for grids whose full host arrays would not fit:
+ * opening draws everything from the seed and makes two passes over the volume (interior
+ * maxima and edge planes of the boundary, then the CFL velocity of the finished model);
+ * fl_model_stream_fill then writes planes z0 .. z0+nzs-1 of the six arrays (slab arrays of
+ * nzs planes), boundary applied, bit for bit the planes of fl_model_gradient /
+ * fl_model_layered.  Slabs come in increasing z from plane 0; after the last plane the
+ * stream starts over.  The fill has the signature of fl_model_slab_fn (fl_hip.h), with the
+ * stream as its user pointer.  NULL when a host allocation fails. */
+typedef struct fl_model_stream fl_model_stream_t;
+fl_model_stream_t *fl_model_stream_gradient(const fl_gradient_model_t *m, int sx, int sy, int sz, int nx, int ny,
+                                            int nz, int bord, int absorb);
+fl_model_stream_t *fl_model_stream_layered(const fl_layered_model_t *m, int sx, int sy, int sz, int nx, int ny,
+                                           int nz, int bord, int absorb);
+int fl_model_stream_fill(void *stream, int z0, int nzs, float *vpz, float *vsv, float *epsilon, float *delta,
+                         float *phi, float *theta);
+/* fl_cfl_dt of the finished model. */
+float fl_model_stream_cfl_dt(const fl_model_stream_t *s, float h, float courant);
+void fl_model_stream_close(fl_model_stream_t *s);
 
 /* dt from a Courant number on max(vpz*sqrt(1+2*eps)) over the array. */
 float fl_cfl_dt(int64_t n, const float *vpz, const float *epsilon, float h, float courant);

@@ -438,3 +438,31 @@ def test_literal_config1_model_matches_reference_backend(fl, oracle):
     assert np.array_equal(r["pc"], R.fetch("pc")) and np.array_equal(r["qc"], R.fetch("qc"))
     assert float(np.abs(r["pc"]).max()) > 0
 
+
+
+@pytest.mark.parametrize("kind,n3,absorb,slabs", [("gradient", (20, 18, 16), 4, (1, 7, 100)),
+                                                  ("layered", (24, 20, 18), 6, (3, 50)),
+                                                  ("layered", (16, 12, 10), 32, (5, 200))])
+def test_streamed_model_equals_whole_model(fl, kind, n3, absorb, slabs):
+    """The model generated in z slabs (boundary applied slab by slab, rand() stream carried
+    across slabs) equals the whole-volume generator bit for bit, for every slab size, and
+    twice in a row; its CFL step equals fl_cfl_dt of the whole model."""
+    gen = fl.model_gradient if kind == "gradient" else fl.model_layered
+    whole = gen(n3, absorb, seed=5, smooth_smin=4.0, smooth_smax=10.0)
+    s = fl.ModelStream(kind, n3, absorb, 5, smooth_smin=4.0, smooth_smax=10.0)
+    try:
+        assert s.shape == whole[0].shape
+        sz = s.shape[0]
+        for slab in slabs:
+            for _ in range(2):
+                got = [[] for _ in range(6)]
+                for z0 in range(0, sz, slab):
+                    for k, a in enumerate(s.fill(z0, min(slab, sz - z0))):
+                        got[k].append(a)
+                for k in range(6):
+                    assert np.array_equal(np.concatenate(got[k]), whole[k]), (slab, k)
+        assert s.cfl_dt(12.5, 0.4) == fl.cfl_dt(whole, 12.5, 0.4)
+        with pytest.raises(fl.FletcherError):
+            s.fill(3, 2)  # slabs must continue where the last one ended
+    finally:
+        s.close()

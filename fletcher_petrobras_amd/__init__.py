@@ -68,6 +68,8 @@ def lib():
         "fl_set_sync": (i, [i]),
         "fl_set_tiles": (i, [i, i]),
         "fl_set_stream_hints": (i, [i]),
+        "fl_set_tile_strip": (i, [i]),
+        "fl_get_tile_strip": (i, []),
         "fl_get_stream_hints": (i, []),
         "fl_get_tiles": (i, [C.POINTER(i), C.POINTER(i)]),
         "fl_autotune": (i, [i, C.c_char_p]),
@@ -461,6 +463,13 @@ class Driver:
 
     def stream_hints(self):
         return bool(self.L.fl_get_stream_hints())
+
+    def set_tile_strip(self, sw):
+        """Tile columns per strip of the step's workgroup order (0: the grid's default)."""
+        _check(self.L.fl_set_tile_strip(sw))
+
+    def tile_strip(self):
+        return self.L.fl_get_tile_strip()
 
     def autotune(self, steps=5, cache=None):
         _check(self.L.fl_autotune(steps, cache.encode() if cache else None))

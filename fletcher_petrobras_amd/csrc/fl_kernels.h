@@ -102,6 +102,7 @@ struct StepArgs {
     float vq;             // CM_Q16R: 2^ebase, the unit of the block exponent 0
     int sx, sy, sz, bord, px;
     int ntx, nty, zt, nblk;  // tiles along x, y; z chunk depth; logical blocks
+    int sw;                  // tile columns per strip of the workgroup order (tile_of)
     int xorg;                // x of the first tile column (0 or bord)
     // step index it: it_host for a plain launch.  Inside a multi-step run, run points at
     // the run's control block {k0, k1, it_base, src_n}: the step reads its counter
@@ -274,6 +275,22 @@ __device__ __forceinline__ void combine(const Coef &k, float dt, float pxx, floa
 __device__ __forceinline__ int xcd_remap(int b, int g)
 {
     return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+// Tile (tx, ty, tz) of logical block t: z chunks outermost; inside a chunk, strips of sw
+// tile columns (the last one narrower) walked row by row.  The tiles an XCD holds at once
+// (consecutive t) then form a compact block of the plane instead of part of one long row,
+// so the halos they share meet in that XCD's L2.  sw >= ntx is the plain row-major order.
+__device__ __forceinline__ void tile_of(int t, int ntx, int nty, int sw, int &tx, int &ty, int &tz)
+{
+    const int per = ntx * nty;
+    tz = t / per;
+    const int r = t - tz * per;
+    const int x0 = r / (sw * nty) * sw;
+    const int w = min(sw, ntx - x0);
+    const int u = r - x0 * nty;
+    ty = u / w;
+    tx = x0 + (u - ty * w);
 }
 
 // ---------------------------------------------------------------------------------

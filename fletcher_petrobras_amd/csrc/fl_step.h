@@ -602,9 +602,9 @@ void tti_step(const StepArgs a)
         a.run[1 - a.par] = a.run[a.par] + 1;
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     if (t >= a.nblk) return;
-    const int tix = uni(t % a.ntx);
-    const int rest = uni(t / a.ntx);
-    step_tile<TY, CM, TWOPASS, ISO, HINTS ? 2 : 0>(a, ctl, tix, uni(rest % a.nty), uni(rest / a.nty), s);
+    int tx, ty, tz;
+    tile_of(t, a.ntx, a.nty, a.sw, tx, ty, tz);
+    step_tile<TY, CM, TWOPASS, ISO, HINTS ? 2 : 0>(a, ctl, uni(tx), uni(ty), uni(tz), s);
 }
 
 // Shot of a batched run (device copy of the host descriptor).

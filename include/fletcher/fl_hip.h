@@ -75,6 +75,14 @@ int fl_get_tiles(int *ty, int *zt);
 int fl_set_stream_hints(int on);
 int fl_get_stream_hints(void);
 
+/* Workgroup order of the z-marching step: strips of sw tile columns, each walked row by row,
+ * so that the tiles one XCD runs at once form a compact block whose shared halos meet in its
+ * L2.  0 (the default) takes 8 columns (or all of them on narrower grids); a width at or
+ * above the tile columns is the plain row-major order.  Results
+ * are identical for every width.  fl_get_tile_strip returns the width in use. */
+int fl_set_tile_strip(int sw);
+int fl_get_tile_strip(void);
+
 /* Time every candidate (ty, zt), with and without stream hints, for
  * `steps` steps on the initialised grid and keep the fastest; fields are restored
  * afterwards.  When cache_path is not NULL the choice is looked up / appended there, keyed

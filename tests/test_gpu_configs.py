@@ -38,6 +38,7 @@ def drv(fl):
         d.finalize()
     z = fl.Driver(variant="fused", precision="fp32", sync=True, tiles=(12, 0))
     z.set_stream_hints(False)
+    z.set_tile_strip(0)
     z.finalize()
 
 

@@ -30,6 +30,7 @@ def drv(fl):
         d.finalize()
     z = fl.Driver(variant="fused", precision="fp32", sync=True, tiles=(12, 0))
     z.set_stream_hints(False)
+    z.set_tile_strip(0)
     z.finalize()
 
 
@@ -204,6 +205,32 @@ def test_tile_shapes_are_bitwise_identical(fl, drv, tiles):
         d.finalize()
     for k in (0, 1):
         assert np.array_equal(out[0][k], out[1][k])
+
+
+@pytest.mark.parametrize("variant,prec,tiles", [("fused", "fp32", (12, 0)), ("fused", "q16r", (8, 11)),
+                                                ("twopass", "fp32", (16, 0))])
+def test_strip_orders_are_bitwise_identical(fl, drv, variant, prec, tiles):
+    """The workgroup order (strips of sw tile columns) moves tiles between workgroups and
+    XCDs only: every width gives the same bits, the plain row-major order (a width at or
+    above the tile columns) included, and the default picks a width within the columns."""
+    m, dt = hetero(fl, (150, 61, 41), 6)
+    shape = m[0].shape
+    isrc = fl.index(shape, 70, 30, 15)
+    out = []
+    for sw in (0, 1, 2, 3, 5, 1000):
+        d = drv(variant=variant, precision=prec, tiles=tiles)
+        d.set_tile_strip(sw)
+        d.initialize(shape, D, dt, m)
+        got = d.tile_strip()
+        assert got == min(sw or 8, 6)  # 6 tile columns of 32 from x = bord
+        d.run(24, isrc, graph_chunk=8)
+        out.append((d.field("pc"), d.field("qc")))
+        d.finalize()
+    for o in out[1:]:
+        for k in (0, 1):
+            assert np.array_equal(out[0][k], o[k])
+    assert fl.lib().fl_set_tile_strip(-1) == -1  # FL_ERR_ARG
+    fl.lib().fl_clear_error()
 
 
 @pytest.mark.parametrize("variant,prec", [("fused", "fp32"), ("fused", "fp32r"), ("fused", "bf16"),

@@ -231,8 +231,9 @@ def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, 
     return r
 
 
-def batched(F, np, precisions, steps, nshots):
-    """Batched shots on apertures cut from one 512x512x320 parent model."""
+def batched(F, np, precisions, steps, nshots, tiles=None):
+    """Batched shots on apertures cut from one 512x512x320 parent model; tiles (ty, zt), or
+    None for the autotuner's choice per precision."""
     from fletcher_petrobras_amd import shots as S
     absorb = 16
     t0 = time.time()
@@ -264,8 +265,11 @@ def batched(F, np, precisions, steps, nshots):
         t0 = time.time()
         b = F.Batch(sh, D, dt, precision=prec)
         try:
-            b.autotune(6)
-            tiles = b.tiles()
+            if tiles is None:
+                b.autotune(6)
+            else:
+                b.set_tiles(*tiles)
+            tl = b.tiles()
             # untimed first run of up to 50 steps: work list upload, graph capture
             b.run(min(steps, 50), graph_chunk=50, rec_every=4, max_traces=(steps + 3) // 4,
                   src=src[:min(steps, 50)])
@@ -278,7 +282,7 @@ def batched(F, np, precisions, steps, nshots):
         finally:
             b.destroy()
         value = 1e-6 * samples * steps / (ms * 1e-3)
-        out[prec] = {"value": round(value, 1), "ms_per_step": round(ms / steps, 4), "tiles": list(tiles),
+        out[prec] = {"value": round(value, 1), "ms_per_step": round(ms / steps, 4), "tiles": list(tl),
                      "finite": finite,
                      "traces_rel_l2_vs_fp64": {"shot%d" % k: round(e, 9) for k, e in enumerate(err)}}
         log("batched %s: %.0f MS/s (%.1f ms for %d steps; %.1f s incl. upload and tuning), traces vs FP64 %s"
@@ -444,6 +448,9 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="only the 1024x1024x768 layered model (FP32 + FP16 recompute vs FP64)")
     ap.add_argument("--large-steps", type=int, default=500)
+    ap.add_argument("--batched", default="",
+                    help="only the batched shots, in these precisions (comma list; for profiling)")
+    ap.add_argument("--batch-tiles", default="", help="ty,zt of the batched runs (default: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the Fletcher hot path here is single-GPU "
@@ -454,6 +461,11 @@ def main():
 
     if args.large:
         large(F, np, args)
+        return
+    if args.batched:
+        precs = tuple(args.batched.split(","))
+        bt = tuple(int(v) for v in args.batch_tiles.split(",")) if args.batch_tiles else None
+        print(json.dumps({"batched": batched(F, np, precs, args.batch_steps, args.shots, bt)}), flush=True)
         return
     n = args.nx
     t0 = time.time()

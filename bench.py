@@ -121,10 +121,11 @@ def cpu_baseline(model, shape, dt, isrc, steps, threads):
             "variants": rows, "cpu_model": cpu_model, "nproc": nproc, "cpus_allowed": allowed}
 
 
-def tiles_for(d, spec):
-    """(ty, zt, stream hints): the autotuner's choice for 'auto', else 'ty,zt[,hints]'."""
+def tiles_for(d, spec, tune_steps=10):
+    """(ty, zt, stream hints): the autotuner's choice for 'auto' (tune_steps steps a
+    candidate), else 'ty,zt[,hints]'."""
     if spec == "auto":
-        ty, zt = d.autotune(10)
+        ty, zt = d.autotune(tune_steps)
         return ty, zt, int(d.stream_hints())
     v = [int(x) for x in spec.split(",")]
     d.set_tiles(v[0], v[1])
@@ -232,8 +233,8 @@ def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, 
 
 
 def batched(F, np, precisions, steps, nshots, tiles=None):
-    """Batched shots on apertures cut from one 512x512x320 parent model; tiles (ty, zt), or
-    None for the autotuner's choice per precision."""
+    """Batched shots on apertures cut from one 512x512x320 parent model, records computed on
+    the device; tiles (ty, zt), or None for the autotuner's choice per precision."""
     from fletcher_petrobras_amd import shots as S
     absorb = 16
     t0 = time.time()
@@ -263,13 +264,19 @@ def batched(F, np, precisions, steps, nshots, tiles=None):
             r.destroy()
     for prec in precisions:
         t0 = time.time()
-        b = F.Batch(sh, D, dt, precision=prec)
+        F.lib().fl_set_device_precompute(1)  # the shots' records computed on the device
+        try:
+            b = F.Batch(sh, D, dt, precision=prec)
+        finally:
+            F.lib().fl_set_device_precompute(0)
+        t_up = time.time() - t0
         try:
             if tiles is None:
                 b.autotune(6)
             else:
                 b.set_tiles(*tiles)
             tl = b.tiles()
+            t_tune = time.time() - t0 - t_up
             # untimed first run of up to 50 steps: work list upload, graph capture
             b.run(min(steps, 50), graph_chunk=50, rec_every=4, max_traces=(steps + 3) // 4,
                   src=src[:min(steps, 50)])
@@ -285,8 +292,8 @@ def batched(F, np, precisions, steps, nshots, tiles=None):
         out[prec] = {"value": round(value, 1), "ms_per_step": round(ms / steps, 4), "tiles": list(tl),
                      "finite": finite,
                      "traces_rel_l2_vs_fp64": {"shot%d" % k: round(e, 9) for k, e in enumerate(err)}}
-        log("batched %s: %.0f MS/s (%.1f ms for %d steps; %.1f s incl. upload and tuning), traces vs FP64 %s"
-            % (prec, value, ms, steps, time.time() - t0, ["%.2e" % e for e in err]))
+        log("batched %s: %.0f MS/s (%.1f ms for %d steps; %.1f s in all: upload %.1f s, tuning %.1f s), "
+            "traces vs FP64 %s" % (prec, value, ms, steps, time.time() - t0, t_up, t_tune, ["%.2e" % e for e in err]))
     return out
 
 
@@ -311,14 +318,14 @@ def window_model(F, np, stream, corner, w, absorb):
     return out
 
 
-def large(F, np, args):
+def large(F, np, args, tune_steps=3):
     """The 1024x1024x768 layered model (array 1096x1096x840, absorb 32, dx=dy=dz=10 m, Courant
     0.4): produced in z slabs and initialised on the device (fl_initialize_streamed), so no
     whole model arrays exist on the host.  FP32 drop-in through the per-step API, and 50-step
     graph runs of FP32 recompute, FP16 recompute and block floating point; then the error of
     each against FP64 on a 256^3 sub-window of the model with its own absorbing zone: the
     device FP64 reference over all steps, and the FP64 oracle on the host over the first 40
-    steps (which the device reference must equal bit for bit)."""
+    steps (which the device reference must equal bit for bit).  Returns the result line."""
     d3 = (10.0, 10.0, 10.0)
     absorb = 32
     t0 = time.time()
@@ -336,7 +343,7 @@ def large(F, np, args):
     t0 = time.time()
     d.initialize_streamed(stream, d3, dt)
     log("streamed initialisation %.1f s" % (time.time() - t0))
-    ty, zt, hints = tiles_for(d, args.tiles)
+    ty, zt, hints = tiles_for(d, args.tiles, tune_steps)
     bpc = d.bytes_per_cell()
     src = F.sources(dt, steps)
     for it in range(1, 6):
@@ -365,7 +372,7 @@ def large(F, np, args):
     for prec in ("fp32r", "fp16r", "q16r"):
         g = F.Driver(variant="fused", precision=prec, sync=False)
         g.initialize_streamed(stream, d3, dt)
-        gty, gzt, gh = tiles_for(g, args.tiles)
+        gty, gzt, gh = tiles_for(g, args.tiles, tune_steps)
         g.run(min(steps, 50), isrc, graph_chunk=50, src=src[:min(steps, 50)])
         g.reset_fields()
         g.synchronize()
@@ -427,7 +434,7 @@ def large(F, np, args):
         "graph_runs": graphs,
         "fp64_window": window,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def main():
@@ -448,6 +455,7 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="only the 1024x1024x768 layered model (FP32 + FP16 recompute vs FP64)")
     ap.add_argument("--large-steps", type=int, default=500)
+    ap.add_argument("--no-large", action="store_true", help="skip the 1024x1024x768 section of the full run")
     ap.add_argument("--batched", default="",
                     help="only the batched shots, in these precisions (comma list; for profiling)")
     ap.add_argument("--batch-tiles", default="", help="ty,zt of the batched runs (default: autotuned)")
@@ -460,7 +468,7 @@ def main():
     import fletcher_petrobras_amd as F
 
     if args.large:
-        large(F, np, args)
+        print(json.dumps(large(F, np, args)), flush=True)
         return
     if args.batched:
         precs = tuple(args.batched.split(","))
@@ -504,6 +512,13 @@ def main():
         for prec in ("fp32", "bf16", "fp16r", "q16r"):
             bt[prec]["vs_fp32_dropin"] = round(bt[prec]["value"] / value, 3)
         extras["batched"] = bt
+        if not args.no_large:
+            t0 = time.time()
+            lg = large(F, np, args)
+            extras["large_1024x1024x768"] = {k: lg[k] for k in ("metric", "value", "unit", "steps", "ms_per_step",
+                                                                  "config", "roofline", "finite", "graph_runs",
+                                                                  "fp64_window")}
+            log("large model section %.1f s" % (time.time() - t0))
 
     cpu = None
     if not args.no_cpu:

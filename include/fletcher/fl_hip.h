@@ -103,9 +103,9 @@ typedef int (*fl_model_slab_fn)(void *user, int z0, int nzs, float *vpz, float *
 int fl_initialize_streamed(int sx, int sy, int sz, int bord, float dx, float dy, float dz, float dt,
                            fl_model_slab_fn fill, void *user);
 
-/* 1: DRIVER_Initialize computes the coefficient records on the device as well (FP64 trig
- * of the device library; records equal to the host's wherever the two libraries' sine and
- * cosine agree).  0 (default): host precompute, bit for bit precomp.h. */
+/* 1: DRIVER_Initialize and fl_batch_create compute the coefficient records on the device
+ * as well (FP64 trig of the device library; records equal to the host's wherever the two
+ * libraries' sine and cosine agree).  0 (default): host precompute, bit for bit precomp.h. */
 int fl_set_device_precompute(int on);
 
 /* Device fields after the last step: which = 0 pp, 1 pc, 2 qp, 3 qc (n floats). */

@@ -176,3 +176,33 @@ def test_batch_oracle_parity(fl, oracle):
     for k, s in enumerate(shots):
         r = oracle.run("der1der1", s["shape"], 4, D, dt, st, s["isource"], oracle.precompute(s["model"]))
         assert rel_l2(got[k], r["pc"]) <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16r", "q16r"])
+def test_batch_device_precompute_equals_single_shots(fl, prec):
+    """Batch records computed on the device (fl_set_device_precompute) give every shot the
+    bits of its single-shot run with device records too."""
+    shots, dt = _batch_setup(fl, nshots=2)
+    st, every = 24, 4
+    fl.lib().fl_set_device_precompute(1)
+    try:
+        b = fl.Batch(shots, D, dt, precision=prec)
+        try:
+            b.run(st, graph_chunk=8, rec_every=every, max_traces=st // every)
+            got = [(b.field(k, "pc"), b.traces(k)) for k in range(len(shots))]
+        finally:
+            b.destroy()
+        for k, s in enumerate(shots):
+            d = fl.Driver(variant="fused", precision=prec)
+            try:
+                d.initialize(s["shape"], D, dt, s["model"])
+                d.record_traces(s["rec_iz"], every, st // every)
+                d.run(st, s["isource"], graph_chunk=0)
+                ref = (d.field("pc"), d.traces())
+            finally:
+                d.finalize()
+            for x, y in zip(got[k], ref):
+                assert np.abs(y).max() > 0
+                assert np.array_equal(x, y), (k, rel_l2(x, y))
+    finally:
+        fl.lib().fl_set_device_precompute(0)

@@ -590,7 +590,15 @@ static int launch_step(StepArgs a, int it_host, hipStream_t s)
         d.dyzinv = 1.0f / (D.dy * D.dz);
         d.dt = D.dt;
         d.sx = D.sx; d.sy = D.sy; d.sz = D.sz; d.bord = D.bord;
-        d.zt = 32;
+        // z chunks: about four rounds of the resident workgroups, chunks of 16 planes or more
+        d.ntx = (D.sx - 2 * D.bord + DX_TX - 1) / DX_TX;
+        d.nty = (D.sy - 2 * D.bord + DX_TY - 1) / DX_TY;
+        const int nzu = D.sz - 2 * D.bord;
+        const long long layer = (long long)d.ntx * d.nty;
+        const int nzc = (int)std::max<long long>(
+            1, std::min<long long>((4ll * DX_PER_CU * std::max(g_cus, 1) + layer - 1) / layer, nzu / 16));
+        d.zt = (nzu + nzc - 1) / nzc;
+        d.nblk = d.ntx * d.nty * ((nzu + d.zt - 1) / d.zt);
         d.trace = nullptr;
         d.rec_iz = D.rec_iz;
         if (D.trace && D.rec_every > 0 && it_host % D.rec_every == 0) {
@@ -598,9 +606,7 @@ static int launch_step(StepArgs a, int it_host, hipStream_t s)
             if (slot >= 0 && slot < D.rec_slots)
                 d.trace = D.trace + (long long)slot * (D.sx - 2 * D.bord) * (D.sy - 2 * D.bord);
         }
-        dim3 g((D.sx - 2 * D.bord + 63) / 64, (D.sy - 2 * D.bord + 3) / 4,
-               (D.sz - 2 * D.bord + d.zt - 1) / d.zt);
-        hipLaunchKernelGGL(direct_dercross, g, dim3(64, 4), 0, s, d);
+        hipLaunchKernelGGL(direct_dercross, dim3((d.nblk + 7) / 8 * 8), dim3(64, DX_TY), 0, s, d);
         return FL_OK;
     }
     return fail(FL_ERR_ARG, "unknown variant %d", g_variant);

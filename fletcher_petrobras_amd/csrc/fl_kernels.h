@@ -297,37 +297,6 @@ __device__ __forceinline__ void tile_of(int t, int ntx, int nty, int sw, int &tx
 // Kernels launched by the driver translation unit only (the kernel instantiation units
 // define FL_NO_AUX_KERNELS).
 #ifndef FL_NO_AUX_KERNELS
-// Direct 64-point DerCross form (original/sample.h:32-90), global loads through L1/L2.
-__device__ __forceinline__ float cross_group(const float *__restrict__ p, long long i, long long s11,
-                                             long long s21, int m, int n)
-{
-    const long long aa = n * s11, bb = m * s21;
-    float v = p[i + bb + aa] - p[i + bb - aa] - p[i - bb + aa] + p[i - bb - aa];
-    if (m != n) {
-        const long long a2 = m * s11, b2 = n * s21;
-        v = v + p[i + b2 + a2] - p[i + b2 - a2] - p[i - b2 + a2] + p[i - b2 - a2];
-    }
-    return v;
-}
-
-__device__ __forceinline__ float dercross_g(const float *__restrict__ p, long long i, long long s11,
-                                            long long s21, float dinv)
-{
-    return (FL_L11 * cross_group(p, i, s11, s21, 1, 1) + FL_L12 * cross_group(p, i, s11, s21, 1, 2) +
-            FL_L13 * cross_group(p, i, s11, s21, 1, 3) + FL_L14 * cross_group(p, i, s11, s21, 1, 4) +
-            FL_L22 * cross_group(p, i, s11, s21, 2, 2) + FL_L23 * cross_group(p, i, s11, s21, 2, 3) +
-            FL_L24 * cross_group(p, i, s11, s21, 2, 4) + FL_L33 * cross_group(p, i, s11, s21, 3, 3) +
-            FL_L34 * cross_group(p, i, s11, s21, 3, 4) + FL_L44 * cross_group(p, i, s11, s21, 4, 4)) *
-           dinv;
-}
-
-__device__ __forceinline__ float der2_g(const float *__restrict__ p, long long i, long long s, float d2inv)
-{
-    return (FL_K0 * p[i] + FL_K1 * (p[i + s] + p[i - s]) + FL_K2 * (p[i + 2 * s] + p[i - 2 * s]) +
-            FL_K3 * (p[i + 3 * s] + p[i - 3 * s]) + FL_K4 * (p[i + 4 * s] + p[i - 4 * s])) *
-           d2inv;
-}
-
 __device__ __forceinline__ float der1_g(const float *__restrict__ p, long long i, long long s, float dinv)
 {
     return (FL_L1 * (p[i + s] - p[i - s]) + FL_L2 * (p[i + 2 * s] - p[i - 2 * s]) +
@@ -341,45 +310,173 @@ struct DirectArgs {
     const float *coef;    // FP32 records of 10 per cell
     float dxxinv, dyyinv, dzzinv, dxyinv, dxzinv, dyzinv, dt;
     int sx, sy, sz, bord, px, zt;
+    int ntx, nty, nblk;   // tiles of 64 x DX_TY cells along x, y; z chunks of zt planes
     float *trace;
     int rec_iz;
 };
 
-// grid: (ceil(nx/64), ceil(ny/4), nzchunks); block (64, 4).  Fields are (p, q) pairs:
-// element strides 2 (x), 2 px (y), 2 px sy (z); p at even, q at odd elements.
-__global__ __launch_bounds__(256) void direct_dercross(const DirectArgs a)
+// Both fields of a cell, so that one expression evaluates the p and the q derivative with
+// the same operations in the same order.
+struct PQ {
+    float p, q;
+};
+__device__ __forceinline__ PQ operator+(PQ a, PQ b) { return {a.p + b.p, a.q + b.q}; }
+__device__ __forceinline__ PQ operator-(PQ a, PQ b) { return {a.p - b.p, a.q - b.q}; }
+__device__ __forceinline__ PQ operator*(float s, PQ a) { return {s * a.p, s * a.q}; }
+__device__ __forceinline__ PQ operator*(PQ a, float s) { return {a.p * s, a.q * s}; }
+
+// Der2 and DerCross of derivatives.h:53,59-68 in the reference's association order, over an
+// accessor: g(m) is the value m cells along the axis, h(a, b) the value a cells along the
+// first axis (s11) and b along the second (s21).
+template <class G>
+__device__ __forceinline__ PQ der2_a(G g, float d2inv)
 {
-    const int x = a.bord + blockIdx.x * 64 + threadIdx.x;
-    const int y = a.bord + blockIdx.y * 4 + threadIdx.y;
-    if (x >= a.sx - a.bord || y >= a.sy - a.bord) return;
-    const int zlo = a.bord + blockIdx.z * a.zt;
-    const int zhi = min(zlo + a.zt, a.sz - a.bord);
-    const long long sX = 2, sY = 2ll * a.px, sZ = 2ll * a.px * a.sy;
-    const float *pc = a.cur, *qc = a.cur + 1;
-    for (int z = zlo; z < zhi; z++) {
-        const long long c = ((long long)z * a.sy + y) * a.px + x, i = 2 * c;
-        const float pxx = der2_g(pc, i, sX, a.dxxinv);
-        const float pyy = der2_g(pc, i, sY, a.dyyinv);
-        const float pzz = der2_g(pc, i, sZ, a.dzzinv);
-        const float pxy = dercross_g(pc, i, sX, sY, a.dxyinv);
-        const float pyz = dercross_g(pc, i, sY, sZ, a.dyzinv);
-        const float pxz = dercross_g(pc, i, sX, sZ, a.dxzinv);
-        const float qxx = der2_g(qc, i, sX, a.dxxinv);
-        const float qyy = der2_g(qc, i, sY, a.dyyinv);
-        const float qzz = der2_g(qc, i, sZ, a.dzzinv);
-        const float qxy = dercross_g(qc, i, sX, sY, a.dxyinv);
-        const float qyz = dercross_g(qc, i, sY, sZ, a.dyzinv);
-        const float qxz = dercross_g(qc, i, sX, sZ, a.dxzinv);
-        Coef k;
+    return (FL_K0 * g(0) + FL_K1 * (g(1) + g(-1)) + FL_K2 * (g(2) + g(-2)) + FL_K3 * (g(3) + g(-3)) +
+            FL_K4 * (g(4) + g(-4))) *
+           d2inv;
+}
+template <class H>
+__device__ __forceinline__ PQ cross_group_a(H h, int m, int n)
+{
+    PQ v = h(n, m) - h(-n, m) - h(n, -m) + h(-n, -m);
+    if (m != n) v = v + h(m, n) - h(-m, n) - h(m, -n) + h(-m, -n);
+    return v;
+}
+template <class H>
+__device__ __forceinline__ PQ dercross_a(H h, float dinv)
+{
+    return (FL_L11 * cross_group_a(h, 1, 1) + FL_L12 * cross_group_a(h, 1, 2) + FL_L13 * cross_group_a(h, 1, 3) +
+            FL_L14 * cross_group_a(h, 1, 4) + FL_L22 * cross_group_a(h, 2, 2) + FL_L23 * cross_group_a(h, 2, 3) +
+            FL_L24 * cross_group_a(h, 2, 4) + FL_L33 * cross_group_a(h, 3, 3) + FL_L34 * cross_group_a(h, 3, 4) +
+            FL_L44 * cross_group_a(h, 4, 4)) *
+           dinv;
+}
+
+// dercross_a as a running sum over the coefficient groups (the same additions in the same
+// order), with compiler barriers after the groups of L13, L23 and L44.
+template <class H>
+__device__ __forceinline__ PQ dercross_runs(H h, float dinv)
+{
+    PQ s = FL_L11 * cross_group_a(h, 1, 1);
+    s = s + FL_L12 * cross_group_a(h, 1, 2);
+    s = s + FL_L13 * cross_group_a(h, 1, 3);
+    asm volatile("" ::: "memory");
+    s = s + FL_L14 * cross_group_a(h, 1, 4);
+    s = s + FL_L22 * cross_group_a(h, 2, 2);
+    s = s + FL_L23 * cross_group_a(h, 2, 3);
+    asm volatile("" ::: "memory");
+    s = s + FL_L24 * cross_group_a(h, 2, 4);
+    s = s + FL_L33 * cross_group_a(h, 3, 3);
+    s = s + FL_L34 * cross_group_a(h, 3, 4);
+    s = s + FL_L44 * cross_group_a(h, 4, 4);
+    asm volatile("" ::: "memory");
+    return s * dinv;
+}
+
+// Direct 64-point DerCross form (original/sample.h:32-90) staged through LDS.  A workgroup
+// of DX_TY waves owns 64 x DX_TY cells of a z chunk; a thread computes both fields of its
+// cell.  The 9 planes of the stencil (tile + 4-cell halo, (p, q) pairs) sit in a ring of
+// DX_NP plane slots; the next plane is stored into the slot that no step of the current plane
+// reads, so one barrier a plane suffices.  Every derivative, the coefficient combination and the
+// leapfrog keep the reference's association order: the results are those of the global-
+// memory form, bit for bit.
+constexpr int DX_TX = 64, DX_TY = 4, DX_NP = 10;
+constexpr int DX_HX = DX_TX + 8, DX_HY = DX_TY + 8, DX_PL = DX_HX * DX_HY;  // pairs per staged plane
+constexpr int DX_NLD = (DX_PL + 64 * DX_TY - 1) / (64 * DX_TY);             // staging loads per thread
+constexpr int DX_PER_CU = (160 * 1024) / (DX_NP * DX_PL * 8);                // resident workgroups a CU
+
+// grid: nblk rounded up to whole XCD rounds; block (64, DX_TY).  Fields are (p, q) pairs at
+// the pitched cell index.
+// DX_PER_CU workgroups a CU (the LDS ring bounds them): DX_PER_CU * DX_TY / 4 waves per SIMD
+__global__ __launch_bounds__(64 * DX_TY) __attribute__((amdgpu_waves_per_eu(DX_PER_CU * DX_TY / 4)))
+void direct_dercross(const DirectArgs a)
+{
+    __shared__ float2 lds[DX_NP][DX_PL];
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    if (t >= a.nblk) return;
+    const int tx = t % a.ntx, rest = t / a.ntx, ty = rest % a.nty, tz = rest / a.nty;
+    const int x0 = a.bord + tx * DX_TX, y0 = a.bord + ty * DX_TY;
+    const int zlo = a.bord + tz * a.zt, zhi = min(zlo + a.zt, a.sz - a.bord);
+    const int f0 = zlo - 4;  // first staged plane
+    const int c = threadIdx.x, r = threadIdx.y, tid = r * 64 + c;
+    const int x = x0 + c, y = y0 + r;
+    const bool active = x < a.sx - a.bord && y < a.sy - a.bord;
+    const long long pxy = (long long)a.px * a.sy;
+    const float2 *cur2 = (const float2 *)a.cur;
+
+    // staging: pair e of a staged plane is (row e / DX_HX, column e % DX_HX) of the window
+    // whose corner is (x0 - 4, y0 - 4); cells outside the array read as zero
+    int goff[DX_NLD];  // pitched cell offset within a plane (planes hold < 2^31 cells)
+    bool gok[DX_NLD];
 #pragma unroll
-        for (int j = 0; j < 10; j++) k.c[rec_order32(j)] = a.coef[c * 10 + j];
-        float ppv = a.prev[i], qpv = a.prev[i + 1];
-        combine<CM_F32>(k, a.dt, pxx, pyy, pzz, pxy, pyz, pxz, qxx, qyy, qzz, qxy, qyz, qxz,
-                        pc[i], qc[i], ppv, qpv);
-        if (a.trace && z == a.rec_iz)
-            a.trace[(long long)(y - a.bord) * (a.sx - 2 * a.bord) + (x - a.bord)] = ppv;
-        a.prev[i] = ppv;
-        a.prev[i + 1] = qpv;
+    for (int k = 0; k < DX_NLD; k++) {
+        const int e = k * 64 * DX_TY + tid;
+        const int row = e / DX_HX, col = e - row * DX_HX;
+        const int gx = x0 - 4 + col, gy = y0 - 4 + row;
+        gok[k] = e < DX_PL && gx < a.sx && gy < a.sy;
+        goff[k] = gy * a.px + gx;
+    }
+    auto fetch = [&](int zp, float2 *v) {
+        const float2 *pl = cur2 + zp * pxy;
+#pragma unroll
+        for (int k = 0; k < DX_NLD; k++) v[k] = (gok[k] && zp < a.sz) ? pl[goff[k]] : make_float2(0.f, 0.f);
+    };
+    auto stage = [&](int slot, const float2 *v) {
+#pragma unroll
+        for (int k = 0; k < DX_NLD; k++) {
+            const int e = k * 64 * DX_TY + tid;
+            if (e < DX_PL) lds[slot][e] = v[k];
+        }
+    };
+    for (int i = 0; i < 9; i++) {
+        float2 v[DX_NLD];
+        fetch(f0 + i, v);
+        stage(i, v);
+    }
+    __syncthreads();
+
+    const int own = (r + 4) * DX_HX + (c + 4);  // own cell in a staged plane
+    for (int z = zlo; z < zhi; z++) {
+        const int j = z - zlo;
+        if (active) {
+            const float2 *pl[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) pl[k] = &lds[(j + k) % DX_NP][own];
+            // both fields at once (8-byte LDS reads), one derivative after the other and the
+            // cross derivatives in three runs of coefficient groups, with compiler barriers
+            // between them: at most 24 pairs of stencil values are live at once
+            auto at = [](const float2 *b, int o) { const float2 v = b[o]; return PQ{v.x, v.y}; };
+            const PQ xx = der2_a([&](int m) { return at(pl[4], m); }, a.dxxinv);
+            const PQ yy = der2_a([&](int m) { return at(pl[4], m * DX_HX); }, a.dyyinv);
+            const PQ zz = der2_a([&](int m) { return at(pl[4 + m], 0); }, a.dzzinv);
+            asm volatile("" ::: "memory");
+            const PQ xy = dercross_runs([&](int u, int v) { return at(pl[4], u + v * DX_HX); }, a.dxyinv);
+            const PQ yz = dercross_runs([&](int u, int v) { return at(pl[4 + v], u * DX_HX); }, a.dyzinv);
+            const PQ xz = dercross_runs([&](int u, int v) { return at(pl[4 + v], u); }, a.dxzinv);
+            const long long cell = (long long)z * pxy + (long long)y * a.px + x;
+            const float2 *rec = (const float2 *)(a.coef + cell * 10);
+            Coef k;
+#pragma unroll
+            for (int w = 0; w < 5; w++) {
+                const float2 v = rec[w];
+                k.c[rec_order32(2 * w)] = v.x;
+                k.c[rec_order32(2 * w + 1)] = v.y;
+            }
+            float2 *pv = (float2 *)a.prev + cell;
+            float2 o = *pv;
+            const PQ cc = {pl[4][0].x, pl[4][0].y};
+            combine<CM_F32>(k, a.dt, xx.p, yy.p, zz.p, xy.p, yz.p, xz.p, xx.q, yy.q, zz.q, xy.q, yz.q, xz.q,
+                            cc.p, cc.q, o.x, o.y);
+            if (a.trace && z == a.rec_iz)
+                a.trace[(long long)(y - a.bord) * (a.sx - 2 * a.bord) + (x - a.bord)] = o.x;
+            *pv = o;
+        }
+        if (z + 5 <= zhi + 3) {  // the next plane into the slot no step of this plane reads
+            float2 nxt[DX_NLD];
+            fetch(z + 5, nxt);
+            stage((j + 9) % DX_NP, nxt);
+        }
+        __syncthreads();
     }
 }
 

@@ -297,12 +297,6 @@ __device__ __forceinline__ void tile_of(int t, int ntx, int nty, int sw, int &tx
 // Kernels launched by the driver translation unit only (the kernel instantiation units
 // define FL_NO_AUX_KERNELS).
 #ifndef FL_NO_AUX_KERNELS
-__device__ __forceinline__ float der1_g(const float *__restrict__ p, long long i, long long s, float dinv)
-{
-    return (FL_L1 * (p[i + s] - p[i - s]) + FL_L2 * (p[i + 2 * s] - p[i - 2 * s]) +
-            FL_L3 * (p[i + 3 * s] - p[i - 3 * s]) + FL_L4 * (p[i + 4 * s] - p[i - 4 * s])) *
-           dinv;
-}
 
 struct DirectArgs {
     float *prev;
@@ -485,6 +479,18 @@ void direct_dercross(const DirectArgs a)
 // Dy over iy in [bord, sy-bord), ix in [bord, sx); every plane iz in [0, sz).
 // dvx, dvy: (pDx, qDx), (pDy, qDy) pairs at the pitched cell index.
 // grid: (ceil(sx/64), ceil(sy/4), sz); block (64, 4)
+// First derivative of both fields of the pair array along a stride of s cells, in the
+// association order of derivatives.h:47.
+__device__ __forceinline__ PQ der1_pq(const float2 *__restrict__ c2, long long i, long long s, float dinv)
+{
+    auto g = [&](long long m) {
+        const float2 v = c2[i + m * s];
+        return PQ{v.x, v.y};
+    };
+    return (FL_L1 * (g(1) - g(-1)) + FL_L2 * (g(2) - g(-2)) + FL_L3 * (g(3) - g(-3)) + FL_L4 * (g(4) - g(-4))) *
+           dinv;
+}
+
 __global__ __launch_bounds__(256) void der1_pass(const float *__restrict__ cur, float *dvx, float *dvy,
                                                  int sx, int sy, int px, int bord, float dxinv,
                                                  float dyinv)
@@ -493,14 +499,15 @@ __global__ __launch_bounds__(256) void der1_pass(const float *__restrict__ cur, 
     const int y = blockIdx.y * 4 + threadIdx.y;
     const int z = blockIdx.z;
     if (x >= sx || y >= sy) return;
-    const long long i = 2 * (((long long)z * sy + y) * px + x);
+    const long long c = ((long long)z * sy + y) * px + x;  // pitched cell: pairs at 2c
+    const float2 *c2 = (const float2 *)cur;
     if (x >= bord && x < sx - bord) {
-        dvx[i] = der1_g(cur, i, 2, dxinv);
-        dvx[i + 1] = der1_g(cur + 1, i, 2, dxinv);
+        const PQ d = der1_pq(c2, c, 1, dxinv);
+        ((float2 *)dvx)[c] = make_float2(d.p, d.q);
     }
     if (y >= bord && y < sy - bord && x >= bord) {
-        dvy[i] = der1_g(cur, i, 2ll * px, dyinv);
-        dvy[i + 1] = der1_g(cur + 1, i, 2ll * px, dyinv);
+        const PQ d = der1_pq(c2, c, px, dyinv);
+        ((float2 *)dvy)[c] = make_float2(d.p, d.q);
     }
 }
 

@@ -410,3 +410,27 @@ def test_errors_are_reported_not_fatal(fl, drv):
                                   None, None, None, None)
         d._status()
     d.step_loop(2, fl.center_index(shape))  # still usable
+
+
+@pytest.mark.parametrize("variant", ["fused", "direct", "twopass"])
+@pytest.mark.parametrize("n3", [(1, 1, 1), (3, 2, 5), (33, 1, 2)])
+def test_smallest_grids(fl, oracle, drv, variant, n3):
+    """Interiors of one to a few cells with a one-cell absorbing zone (update domain of 3 to 35
+    cells a side, single partial tiles and chunks): the same fields as the oracle."""
+    shape = fl.grid_shape(n3[0], n3[1], n3[2], 1)
+    m = fl.model_homogeneous(2, shape)
+    fl.random_boundary(m, n3, 1)
+    isrc = fl.center_index(shape)
+    st = 6
+    d = drv(variant=variant)
+    d.initialize(shape, D, 0.001, m)
+    d.run(st, isrc, graph_chunk=0)
+    ov = "original" if variant == "direct" else "der1der1"
+    r = oracle.run(ov, shape, 4, D, 0.001, st, isrc, oracle.precompute(m))
+    assert np.abs(r["pc"]).max() > 0
+    for k in ("pc", "qc"):
+        got = d.field(k)
+        if variant == "direct":
+            assert np.array_equal(got, r[k]), k
+        else:
+            assert rel_l2(got, r[k]) <= TOL, (k, rel_l2(got, r[k]))

@@ -200,6 +200,26 @@ def fp64_reference(F, np, model, shape, dt, isrc, steps, extra, rec_iz, d3=D):
     return out
 
 
+def pin_fp64_reference(F, np, model, shape, dt, isrc, steps, rec_iz, d3=D):
+    """The device FP64 reference against the FP64 oracle on the host over the first `steps`
+    steps of the same grid and model: pc and traces equal bit for bit."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t0 = time.time()
+    src = F.sources(dt, steps)
+    ro = O.run("der1der1", shape, 4, d3, dt, steps, isrc, O.precompute(model, "f64"), fp="f64", rec_iz=rec_iz,
+               rec_every=2, src=src)
+    r = F.Ref64(shape, d3, dt, model)
+    try:
+        r.run(steps, isrc, src=src, rec_iz=rec_iz, rec_every=2, max_traces=steps // 2)
+        same = bool(np.array_equal(r.field("pc"), ro["pc"]) and np.array_equal(r.traces(), ro["traces"]))
+    finally:
+        r.destroy()
+    log("FP64 oracle on %s, %d steps: %.1f s; device FP64 reference equal bit for bit: %s"
+        % (shape, steps, time.time() - t0, same))
+    return same
+
+
 def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, tiles, ref=None, d3=D):
     """steps time steps from rest in 50-step graph chunks, receivers every 4 steps; the timed
     run replays the graph that a first, untimed run of up to 50 steps captured and
@@ -398,19 +418,7 @@ def large(F, np, args, tune_steps=3):
     log("256^3 window model %s built in %.1f s" % (wshape, time.time() - t0))
     ref = fp64_reference(F, np, wm, wshape, dt, wsrc, steps, 0, rec_iz, d3)
     nchk = min(40, steps)
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    t0 = time.time()
-    ro = O.run("der1der1", wshape, 4, d3, dt, nchk, wsrc, O.precompute(wm, "f64"), fp="f64", rec_iz=rec_iz,
-               rec_every=4, src=src[:nchk])
-    r64 = F.Ref64(wshape, d3, dt, wm)
-    try:
-        r64.run(nchk, wsrc, src=src[:nchk], rec_iz=rec_iz, rec_every=4, max_traces=nchk // 4)
-        ref_pin = bool(np.array_equal(r64.field("pc"), ro["pc"]) and np.array_equal(r64.traces(), ro["traces"]))
-    finally:
-        r64.destroy()
-    log("FP64 oracle on the window, %d steps: %.1f s; device FP64 reference equal bit for bit: %s"
-        % (nchk, time.time() - t0, ref_pin))
+    ref_pin = pin_fp64_reference(F, np, wm, wshape, dt, wsrc, nchk, rec_iz, d3)
     window = {"window": "256^3 interior at (z, y, x) = %s of the model, absorb 32 of its own, array %s, %d steps"
                         % (corner, "x".join(map(str, wshape[::-1])), steps),
               "fp64_reference_equals_oracle_%d_steps" % nchk: ref_pin, "rel_l2_vs_fp64": {}}
@@ -498,6 +506,8 @@ def main():
         main_err = rel_l2(np, pc_main, ref["pc_total"])
         del pc_main
         extras = {"fp32_dropin_rel_l2_vs_fp64": {"pc": round(main_err, 9)},
+                  "fp64_reference_equals_oracle_10_steps": pin_fp64_reference(F, np, model, shape, dt, isrc, 10,
+                                                                               rec_iz),
                   "twopass": graph_run(F, np, model, shape, dt, isrc, "twopass", "fp32", args.steps,
                                        rec_iz, "auto", ref),
                   "fp32_graph": graph_run(F, np, model, shape, dt, isrc, "fused", "fp32", args.steps,

@@ -1,4 +1,4 @@
-"""Multi-shot survey setup: shot sub-grids cut from one parent model.
+"""Multi-shot acquisition setup: shot sub-grids cut from one parent model.
 
 Each shot owns an aperture window of the parent interior; its absorbing zone and halo are
 built the reference way around that window (the window's edge values extended outward, then

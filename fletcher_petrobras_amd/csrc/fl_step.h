@@ -368,9 +368,13 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     float qc_[10], qx_[10], qy_[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) qc_[k] = qx_[k] = qy_[k] = 0.0f;
-    // output-plane streams loaded prefetch_steps() ahead (set = parity of the consuming step)
-    CoefRec<CM> kp[2];
-    float ppv[2] = {0.0f, 0.0f};
+    // output-plane streams loaded prefetch_steps() ahead, in NSETS register sets (the set of
+    // step i is (i - 8) % NSETS; NSETS divides the 10 steps of an unrolled group)
+    constexpr int PDK = prefetch_steps<CM>() > 0 ? prefetch_steps<CM>() : 1;
+    constexpr int NSETS = PDK <= 2 ? 2 : 5;
+    static_assert(PDK < NSETS || PDK == 2, "prefetch distance beyond the register sets");
+    CoefRec<CM> kp[NSETS];
+    float ppv[NSETS] = {};
 
     // prologue: plane f0 into ring slot 0, the next SA planes in flight
     fetch(ic<0>{}, 0);
@@ -385,17 +389,19 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     //   XW   Dx halo planes 0 never / 1 always / 2 if i < nz+4 (f is an output plane)
     //   OUT  output plane z = f-4
     //   GEN  generic copy: plane i-8+k in slot k, queues shifted afterwards
-    //   PF   bit 0: this step's output streams were loaded PD steps earlier into set
-    //        i % 2; bits 1-2: load those of step i + PD into set (i + PD) % 2 (2 always,
-    //        4 if `pf_next`)
-    //   PAR  i % 2 (the register sets of the step)
-    auto step = [&](auto PHc, auto XWc, auto OUTc, auto GENc, auto PFc, auto PARc, const int i,
+    //   PF   bit 0: this step's output streams were loaded PD steps earlier into set PS;
+    //        bits 1-2: load those of step i + PD into set (PS + PD) % NSETS (2 always, 4 if
+    //        `pf_next`)
+    //   PAR  i % 2 (the staging register sets of the step)
+    //   PS   (i - 8) % NSETS (the prefetch register set of the step)
+    auto step = [&](auto PHc, auto XWc, auto OUTc, auto GENc, auto PFc, auto PARc, auto PSc, const int i,
                     const bool pf_next) {
         constexpr int PH = decltype(PHc)::value;
         constexpr int XW = decltype(XWc)::value;
         constexpr bool OUT = decltype(OUTc)::value != 0, GEN = decltype(GENc)::value != 0;
         constexpr int PF = decltype(PFc)::value;
         constexpr int SET = decltype(PARc)::value;
+        constexpr int PS = decltype(PSc)::value;
         constexpr int PD = prefetch_steps<CM>() > 0 ? prefetch_steps<CM>() : 1;
         const int f = f0 + i;
         const int z = f - 4;
@@ -410,8 +416,8 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         CoefRec<CM> kr;
         float vprev = 0.0f;
         if constexpr (OUT && (PF & 1)) {
-            kr = kp[SET];
-            vprev = ppv[SET];
+            kr = kp[PS];
+            vprev = ppv[PS];
         } else if constexpr (OUT) {
             load_rec<CM, PCPOL>(span_rsrc(cpl, pbr), vor, kr);
             cpl += pbr;
@@ -522,7 +528,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         }
         if constexpr ((PF & 2) != 0 || (PF & 4) != 0) {
             if ((PF & 2) || pf_next) {
-                constexpr int NS2 = (SET + PD) & 1;
+                constexpr int NS2 = (PS + PD) % NSETS;
                 load_rec<CM, PCPOL>(span_rsrc(cpl, pbr), vor, kp[NS2]);
                 cpl += pbr;
                 ppv[NS2] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_prev, vo8, (uint32_t)(i + PD - 8) * pb8, PCPOL));
@@ -546,14 +552,21 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     constexpr int PD = prefetch_steps<CM>() > 0 ? prefetch_steps<CM>() : 1;
     constexpr int PFB = prefetch_steps<CM>() ? 4 : 0;
     const bool grp0 = 8 + 9 < nz + 4;
-    step(ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 0, false);
-    step(ic<1>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<1>{}, 1, false);
-    step(ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 2, false);
-    step(ic<3>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<1>{}, 3, false);
-    step(ic<4>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<0>{}, 4, false);
-    step(ic<5>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<0>{}, ic<1>{}, 5, false);
-    step(ic<6>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<(PD == 2 ? PFB : 0)>{}, ic<0>{}, 6, grp0);
-    step(ic<7>{}, ic<2>{}, ic<0>{}, ic<0>{}, ic<PFB>{}, ic<1>{}, 7, grp0);
+    // step j < 8 loads the streams of step j + PD when that is an output step of the first
+    // group; its prefetch set is (j - 8) mod NSETS
+    auto pro = [&](auto Jc, auto XWc) {
+        constexpr int j = decltype(Jc)::value;
+        step(ic<j>{}, XWc, ic<0>{}, ic<0>{}, ic<(j + PD >= 8 ? PFB : 0)>{}, ic<j & 1>{},
+             ic<(j + 10 * NSETS - 8) % NSETS>{}, j, j + PD >= 8 ? grp0 : false);
+    };
+    pro(ic<0>{}, ic<0>{});
+    pro(ic<1>{}, ic<0>{});
+    pro(ic<2>{}, ic<0>{});
+    pro(ic<3>{}, ic<0>{});
+    pro(ic<4>{}, ic<2>{});
+    pro(ic<5>{}, ic<2>{});
+    pro(ic<6>{}, ic<2>{});
+    pro(ic<7>{}, ic<2>{});
     // steady state: groups of 10 planes (i = 8 mod 10 at their start), each writing Dx
     // halos and an output plane, with the output streams PD steps ahead
     int i = 8;
@@ -563,17 +576,18 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             [&](auto P) {
                 constexpr int p = decltype(P)::value;
                 constexpr int pf = prefetch_steps<CM>() ? (1 | (p + PD <= 9 ? 2 : 4)) : 0;
-                step(ic<(8 + p) % 10>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<pf>{}, ic<p & 1>{}, i + p, more);
+                step(ic<(8 + p) % 10>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<pf>{}, ic<p & 1>{}, ic<p % NSETS>{}, i + p,
+                     more);
             },
             std::make_integer_sequence<int, 10>{});
     }
     // the rest (i even here), queues shifted after each plane, two steps per round so that
     // the register sets keep their parity
     for (; i + 1 <= nz + 7; i += 2) {
-        step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<0>{}, i, false);
-        step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<1>{}, i + 1, false);
+        step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<0>{}, ic<0>{}, i, false);
+        step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<1>{}, ic<0>{}, i + 1, false);
     }
-    if (i <= nz + 7) step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<0>{}, i, false);
+    if (i <= nz + 7) step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<0>{}, ic<0>{}, i, false);
 }
 
 // Waves per SIMD the register allocator is asked for.  With 8-row tiles the LDS rings

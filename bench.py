@@ -252,7 +252,7 @@ def graph_run(F, np, model, shape, dt, isrc, variant, precision, steps, rec_iz, 
     return r
 
 
-def batched(F, np, precisions, steps, nshots, tiles=None):
+def batched(F, np, precisions, steps, nshots, tiles=None, persistent=0):
     """Batched shots on apertures cut from one 512x512x320 parent model, records computed on
     the device; tiles (ty, zt), or None for the autotuner's choice per precision."""
     from fletcher_petrobras_amd import shots as S
@@ -291,6 +291,7 @@ def batched(F, np, precisions, steps, nshots, tiles=None):
             F.lib().fl_set_device_precompute(0)
         t_up = time.time() - t0
         try:
+            b.set_persistent(persistent)
             if tiles is None:
                 b.autotune(6)
             else:
@@ -467,6 +468,9 @@ def main():
     ap.add_argument("--batched", default="",
                     help="only the batched shots, in these precisions (comma list; for profiling)")
     ap.add_argument("--batch-tiles", default="", help="ty,zt of the batched runs (default: autotuned)")
+    ap.add_argument("--batch-persistent", type=int, default=0,
+                    help="launch form of the batched step: 0 one workgroup per item, 1 persistent (resident "
+                         "workgroups), >1 persistent over at most that many workgroups")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the Fletcher hot path here is single-GPU "
@@ -481,7 +485,8 @@ def main():
     if args.batched:
         precs = tuple(args.batched.split(","))
         bt = tuple(int(v) for v in args.batch_tiles.split(",")) if args.batch_tiles else None
-        print(json.dumps({"batched": batched(F, np, precs, args.batch_steps, args.shots, bt)}), flush=True)
+        print(json.dumps({"batched": batched(F, np, precs, args.batch_steps, args.shots, bt, args.batch_persistent)}),
+              flush=True)
         return
     n = args.nx
     t0 = time.time()

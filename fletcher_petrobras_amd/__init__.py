@@ -69,6 +69,7 @@ def lib():
         "fl_set_tiles": (i, [i, i]),
         "fl_set_stream_hints": (i, [i]),
         "fl_set_tile_strip": (i, [i]),
+        "fl_batch_set_persistent": (i, [i]),
         "fl_get_tile_strip": (i, []),
         "fl_get_stream_hints": (i, []),
         "fl_get_tiles": (i, [C.POINTER(i), C.POINTER(i)]),
@@ -563,6 +564,11 @@ class Batch:
 
     def set_tiles(self, ty, zt):
         _check(self.L.fl_batch_set_tiles(ty, zt))
+
+    def set_persistent(self, wgs):
+        """0: one workgroup per item; 1: the persistent form over the resident workgroups;
+        > 1: the persistent form over at most wgs workgroups."""
+        _check(self.L.fl_batch_set_persistent(wgs))
 
     def tiles(self):
         ty, zt = C.c_int(), C.c_int()

@@ -58,15 +58,22 @@ int FL_CAT(launch_grid_, FL_TY)(const StepArgs &a, int cm, bool twopass, bool is
 #endif
 
 #ifdef FL_PART_BATCH
+template <int CM, bool P>
+static void bt2(const BatchArgs &a, int grid, bool iso, hipStream_t s)
+{
+    if (iso)
+        hipLaunchKernelGGL((tti_step_batch<FL_TY, CM, true, P>), dim3(grid), dim3(64, FL_TY), 0, s, a, a.shots,
+                           a.item_shot);
+    else
+        hipLaunchKernelGGL((tti_step_batch<FL_TY, CM, false, P>), dim3(grid), dim3(64, FL_TY), 0, s, a, a.shots,
+                           a.item_shot);
+}
+
 template <int CM>
 static void bt(const BatchArgs &a, int grid, bool iso, hipStream_t s)
 {
-    if (iso)
-        hipLaunchKernelGGL((tti_step_batch<FL_TY, CM, true>), dim3(grid), dim3(64, FL_TY), 0, s, a, a.shots,
-                           a.item_shot);
-    else
-        hipLaunchKernelGGL((tti_step_batch<FL_TY, CM, false>), dim3(grid), dim3(64, FL_TY), 0, s, a, a.shots,
-                           a.item_shot);
+    if (a.persist) bt2<CM, true>(a, grid, iso, s);
+    else bt2<CM, false>(a, grid, iso, s);
 }
 
 int FL_CAT(launch_batch_, FL_TY)(const BatchArgs &a, int grid, int cm, bool iso, hipStream_t s)

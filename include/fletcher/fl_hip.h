@@ -174,6 +174,11 @@ typedef struct {
 int fl_batch_create(int nshots, const fl_shot_t *shots, int bord, float dx, float dy, float dz,
                     float dt, int precision);
 int fl_batch_set_tiles(int ty, int zt);
+/* Launch form of the batched step (no reference counterpart).  0 (default): one workgroup
+ * per (shot, tile) item.  Otherwise the persistent form: the resident workgroups (at most
+ * wgs of them when wgs > 1) take the items of their slice of the work list from a ticket
+ * counter.  Results are the same bits either way. */
+int fl_batch_set_persistent(int wgs);
 int fl_batch_get_tiles(int *ty, int *zt);  /* zt as used (0 resolved to the default) */
 /* Steps it0+1..it0+nsteps for every shot; src as in fl_run_steps; traces every
  * rec_every steps (0: none) into max_traces slots per shot. */

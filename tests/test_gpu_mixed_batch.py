@@ -206,3 +206,30 @@ def test_batch_device_precompute_equals_single_shots(fl, prec):
                 assert np.array_equal(x, y), (k, rel_l2(x, y))
     finally:
         fl.lib().fl_set_device_precompute(0)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16r", "q16r"])
+def test_persistent_batch_equals_one_workgroup_per_item(fl, prec):
+    """The persistent form of the batched step (workgroups taking the items of their slice of
+    the work list from a ticket counter) gives the bits of the one-workgroup-per-item launch:
+    with one workgroup a slice (each runs many items in turn, the counters reset after every
+    step, graph replays included) and with the resident workgroups."""
+    shots, dt = _batch_setup(fl)
+    st, every = 30, 3
+    out = []
+    for wgs in (0, 8, 1):
+        b = fl.Batch(shots, D, dt, precision=prec, tiles=(8, 0))
+        try:
+            b.set_persistent(wgs)
+            b.run(st, graph_chunk=10, rec_every=every, max_traces=st // every)
+            out.append([(b.field(k, "pc"), b.field(k, "qc"), b.traces(k)) for k in range(len(shots))])
+        finally:
+            b.set_persistent(0)
+            b.destroy()
+    for alt in out[1:]:
+        for k in range(len(shots)):
+            for x, y in zip(alt[k], out[0][k]):
+                assert np.abs(y).max() > 0
+                assert np.array_equal(x, y), k
+    assert fl.lib().fl_batch_set_persistent(-1) == -1
+    fl.lib().fl_clear_error()

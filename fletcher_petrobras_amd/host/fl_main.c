@@ -3,7 +3,7 @@
  * (original/main.c:47-60):   fletcher FORM nx ny nz absorb dx dy dz dt tmax
  * FORM = ISO | VTI | TTI.  Builds the reference model (main.c:93-260), runs the
  * reference time loop (model.c) on the linked back-end and prints the
- * reference's report lines (main.c:112-119, model.c:130-132).
+ * reference's report lines and Report.csv (main.c:112-119, model.c:117-158).
  *
  * Back-end options come from the environment (the reference has none):
  *   FL_VARIANT=fused|direct|twopass   FL_PRECISION=fp32|bf16|fp16|fp16r|fp32r|q16r
@@ -21,6 +21,36 @@
 #include "fletcher/fl_hip.h"
 #include "fletcher/fl_model.h"
 #include "fletcher/fl_run.h"
+
+/* The process's memory high-water mark (VmHWM of /proc/self/status), as model.c:119-126 reads it. */
+static long high_water_mark(char *unit, int unit_len)
+{
+    long hwm = 0;
+    snprintf(unit, unit_len, "kB");
+    FILE *fp = fopen("/proc/self/status", "r");
+    if (!fp) return 0;
+    char line[256];
+    while (fgets(line, sizeof line, fp))
+        if (strncmp(line, "VmHWM", 5) == 0) {
+            char u[8] = "kB";
+            if (sscanf(line + 6, "%ld %7s", &hwm, u) >= 1) snprintf(unit, unit_len, "%s", u);
+            break;
+        }
+    fclose(fp);
+    return hwm;
+}
+
+/* Report.csv in the layout of model.c:15-28,135-158: problem size, then wall time,
+ * MSamples/s and the high-water mark. */
+static int write_report(int sx, int sy, int sz, int bord, int st, double walltime, double msamples, long hwm,
+                        const char *unit)
+{
+    FILE *fr = fopen("Report.csv", "w");
+    if (!fr) return -1;
+    fprintf(fr, "sx; %d; sy; %d; sz; %d; bord; %d;  st; %d; \n", sx, sy, sz, bord, st);
+    fprintf(fr, "walltime; %lf; MSamples; %lf; HWM;  %ld; HWMUnit;  %s;\n", walltime, msamples, hwm, unit);
+    return fclose(fr);
+}
 
 static int env_choice(const char *name, const char *const *opts, int n, int dflt)
 {
@@ -104,8 +134,13 @@ int main(int argc, char **argv)
         fprintf(stderr, "fletcher: %s\n", fl_last_error());
         return 1;
     }
+    char unit[8];
+    const long hwm = high_water_mark(unit, sizeof unit);
     printf("Execution time (s) is %lf\n", rep.walltime);
     printf("MSamples/s %.0lf\n", rep.msamples);
+    printf("Memory High Water Mark is %ld %s\n", hwm, unit);
+    if (write_report(sx, sy, sz, bord, st, rep.walltime, rep.msamples, hwm, unit) != 0)
+        fprintf(stderr, "fletcher: cannot write Report.csv\n");
     for (int k = 0; k < 6; k++) free(m[k]);
     for (int k = 0; k < 4; k++) free(f[k]);
     return 0;

@@ -4,6 +4,7 @@ same invocation: pc at the start and after every 0.01 s of simulated time
 (der1der1/main.c:273-300, model.c:97-102) in the layout of utils.c:71-205.  The header text
 must be the same bytes; the binary must hold the same number of frames, the first (the zero
 initial field) bitwise equal and each later frame within relative L2 1e-5 of the reference's.
+Report.csv (model.c:135-158) has the reference's lines and keys.
 """
 import os
 import subprocess
@@ -48,3 +49,11 @@ def test_cli_rsf_snapshots_match_reference_dump(fl, ref_dump, variant, tmp_path)
         err = rel_l2(got[k], ref[k])
         print("%s frame %d: relL2 vs reference %.3e" % (variant, k, err))
         assert err <= 1e-5
+    # Report.csv (model.c:135-158): the same problem-size line, the same keys on the metrics line
+    mine = (tmp_path / "Report.csv").read_text().splitlines()
+    theirs = (ref_dir / "Report.csv").read_text().splitlines()
+    assert mine[0] == theirs[0]
+    keys = lambda line: [f.strip() for f in line.split(";")][0::2]
+    assert keys(mine[1]) == keys(theirs[1])
+    assert float(mine[1].split(";")[3]) > 0  # MSamples/s
+    assert "Memory High Water Mark is" in r.stdout

@@ -366,7 +366,7 @@ def large(F, np, args, tune_steps=3):
     log("streamed initialisation %.1f s" % (time.time() - t0))
     ty, zt, hints = tiles_for(d, args.tiles, tune_steps)
     bpc = d.bytes_per_cell()
-    src = F.sources(dt, steps)
+    src = F.sources(dt, max(steps, 5))
     for it in range(1, 6):
         d.insert_source(it - 1, isrc, float(src[it - 1]))
         d.propagate(it)
@@ -389,6 +389,10 @@ def large(F, np, args, tune_steps=3):
     value = 1e-6 * samples * steps / (region_ms * 1e-3)
     achieved = bpc * samples / (avg * 1e-3) / 1e9
     log("large drop-in: %.0f MS/s, kernel %.3f ms, %.0f GB/s, finite=%s" % (value, avg, achieved, finite))
+    if args.large_dropin_only:  # for profiling the FP32 step on this grid
+        stream.close()
+        return {"metric": "large drop-in only", "value": round(value, 1), "kernel_ms": round(avg, 4),
+                "achieved_GBs": round(achieved, 1), "tiles": [ty, zt], "stream_hints": hints, "finite": finite}
     graphs = {}
     for prec in ("fp32r", "fp16r", "q16r"):
         g = F.Driver(variant="fused", precision=prec, sync=False)
@@ -465,6 +469,8 @@ def main():
                     help="only the 1024x1024x768 layered model (FP32 + FP16 recompute vs FP64)")
     ap.add_argument("--large-steps", type=int, default=500)
     ap.add_argument("--no-large", action="store_true", help="skip the 1024x1024x768 section of the full run")
+    ap.add_argument("--large-dropin-only", action="store_true",
+                    help="with --large: only the FP32 drop-in on the 1024x1024x768 grid (for profiling)")
     ap.add_argument("--batched", default="",
                     help="only the batched shots, in these precisions (comma list; for profiling)")
     ap.add_argument("--batch-tiles", default="", help="ty,zt of the batched runs (default: autotuned)")

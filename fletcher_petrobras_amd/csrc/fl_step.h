@@ -208,12 +208,14 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, Coef &
             t = (float)(fld ? (r.w[2] >> 16) : (r.w[2] & 0xffffu)) * (1.0f / 65536.0f);
         } else if constexpr (CM == CM_Q16R) {
             // as CM_F16R, the factors as 15-bit mantissas of the cell's exponent e (bits
-            // 30-31 of the two words) in units of 2^ebase, applied in the leapfrog
+            // 30-31 of the two words) in units of 2^ebase: the mantissas here, 2^e in c[9],
+            // both powers of two applied once in the leapfrog (the same products: scaling by
+            // a power of two is exact)
             const uint32_t e = (r.w[0] >> 30) | ((r.w[1] >> 28) & 0xcu);
-            const float sc = __builtin_bit_cast(float, (e + 127u) << 23);  // 2^e
-            own = (float)((r.w[0] >> (fld ? 15 : 0)) & 0x7fffu) * sc;
-            vpz = (float)(r.w[1] & 0x7fffu) * sc;
-            vsz = (float)((r.w[1] >> 15) & 0x7fffu) * sc;
+            k.c[9] = __builtin_bit_cast(float, (e + 127u) << 23);  // 2^e
+            own = (float)((r.w[0] >> (fld ? 15 : 0)) & 0x7fffu);
+            vpz = (float)(r.w[1] & 0x7fffu);
+            vsz = (float)((r.w[1] >> 15) & 0x7fffu);
             t = (float)(fld ? (r.w[2] >> 16) : (r.w[2] & 0xffffu)) * (1.0f / 65536.0f);
         } else {  // CM_F32R: lane words (own factor, v2pz | v2sz, theta | phi turns)
             own = __builtin_bit_cast(float, r.w[0]);
@@ -234,7 +236,8 @@ __device__ __forceinline__ void decode_rec(const CoefRec<CM> &r, int fld, Coef &
         k.c[3] = st2 * from_qf(sc2);  // sin^2 theta sin 2phi
         k.c[4] = s2t * from_qf(sn);   // sin 2theta sin phi
         k.c[5] = s2t * from_qf(cs);   // sin 2theta cos phi
-        k.c[6] = k.c[9] = own;
+        k.c[6] = own;
+        if constexpr (CM != CM_Q16R) k.c[9] = own;
         k.c[7] = vpz;
         k.c[8] = vsz;
     }
@@ -507,14 +510,14 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             // rhsq = v2pn h2p + v2pz h1q - v2sz (h2p - h2q) = ... + v2sz (h2q - h2p)
             // with h2p, h1q selected per half (original/sample.h:75-81, same roundings)
             const float h2p = fld ? other : h2, h1q = fld ? h1 : other;
-            float rhs = (fld ? kc.c[9] : kc.c[6]) * h2p;
+            float rhs = ((fld && CM != CM_Q16R) ? kc.c[9] : kc.c[6]) * h2p;
             rhs = __builtin_fmaf(kc.c[7], h1q, rhs);
             rhs = __builtin_fmaf(kc.c[8], (fld ? h2 : h1) - other, rhs);
             const float cur = qz[4];
             const float base = 2.0f * cur - vprev;
             float nw;
             if constexpr (CM == CM_F32 || CM == CM_F32R) nw = __builtin_fmaf(rhs * dts, a.dt, base);
-            else if constexpr (CM == CM_Q16R) nw = __builtin_fmaf(rhs, rq, base);
+            else if constexpr (CM == CM_Q16R) nw = __builtin_fmaf(rhs, rq * kc.c[9], base);
             else if constexpr (ISO) nw = __builtin_fmaf(rhs, a.dxxinv, base);
             else nw = base + rhs;
             if (z == rec_z) {

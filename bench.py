@@ -569,9 +569,10 @@ def main():
         "data": "synthetic seeded heterogeneous TTI model (vpz gradient + blobs, smooth eps/delta/theta/phi, "
                 "reference random boundary)",
         "config": {"workload": "TTI %d^3 interior, absorb %d, array %dx%dx%d, dx=12.5 m, dt=%.6g s, "
-                               "%s variant, %s coefficients, tiles 32x%dx%d, stream hints %s"
-                               % (n, args.absorb, shape[2], shape[1], shape[0], dt, args.variant,
-                                  args.precision, ty, zt, "on" if main_r["hints"] else "off"),
+                               "%s variant, %s coefficients, %s"
+                               % (n, args.absorb, shape[2], shape[1], shape[0], dt, args.variant, args.precision,
+                                  "64x4-cell tiles, 10-plane LDS ring" if args.variant == "direct" else
+                                  "tiles 32x%dx%d, stream hints %s" % (ty, zt, "on" if main_r["hints"] else "off")),
                    "samples_per_step": samples, "bytes_per_sample": bpc},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(tag)},

@@ -477,6 +477,9 @@ def main():
     ap.add_argument("--batch-persistent", type=int, default=0,
                     help="launch form of the batched step: 0 one workgroup per item, 1 persistent (resident "
                          "workgroups), >1 persistent over at most that many workgroups")
+    ap.add_argument("--lane-pairs", type=int, default=-1, choices=[-1, 0, 1],
+                    help="lane layout of the fp16r / q16r step: 1 both fields per lane, 0 one field per lane, "
+                         "-1 the library's default")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the Fletcher hot path here is single-GPU "
@@ -485,6 +488,8 @@ def main():
     import numpy as np
     import fletcher_petrobras_amd as F
 
+    if args.lane_pairs >= 0:
+        F._check(F.lib().fl_set_lane_pairs(args.lane_pairs))
     if args.large:
         print(json.dumps(large(F, np, args)), flush=True)
         return

@@ -71,6 +71,8 @@ def lib():
         "fl_set_tile_strip": (i, [i]),
         "fl_batch_set_persistent": (i, [i]),
         "fl_get_tile_strip": (i, []),
+        "fl_set_lane_pairs": (i, [i]),
+        "fl_get_lane_pairs": (i, []),
         "fl_get_stream_hints": (i, []),
         "fl_get_tiles": (i, [C.POINTER(i), C.POINTER(i)]),
         "fl_autotune": (i, [i, C.c_char_p]),
@@ -472,6 +474,13 @@ class Driver:
     def tile_strip(self):
         return self.L.fl_get_tile_strip()
 
+    def set_lane_pairs(self, on):
+        """Both fields of a cell per lane in the 16-bit recompute forms' step (same results)."""
+        _check(self.L.fl_set_lane_pairs(1 if on else 0))
+
+    def lane_pairs(self):
+        return bool(self.L.fl_get_lane_pairs())
+
     def autotune(self, steps=5, cache=None):
         _check(self.L.fl_autotune(steps, cache.encode() if cache else None))
         return self.tiles()
@@ -569,6 +578,11 @@ class Batch:
         """0: one workgroup per item; 1: the persistent form over the resident workgroups;
         > 1: the persistent form over at most wgs workgroups."""
         _check(self.L.fl_batch_set_persistent(wgs))
+
+    def set_lane_pairs(self, on):
+        """Both fields of a cell per lane in the 16-bit recompute forms' step (same results;
+        the setting is the library's, shared with Driver.set_lane_pairs)."""
+        _check(self.L.fl_set_lane_pairs(1 if on else 0))
 
     def tiles(self):
         ty, zt = C.c_int(), C.c_int()

@@ -61,6 +61,7 @@ struct Tiles {
     int ty = 12, zt = 0;  // zt 0: chunk_depth's choice for the grid
     int hints = 0;        // streaming cache hints of the z-marching step (fl_set_stream_hints)
     int sw = 0;           // tile columns per strip (fl_set_tile_strip); 0: strip_width's choice
+    int pair = 1;         // both fields per lane for the forms that have it (fl_set_lane_pairs)
 };
 
 struct Device {
@@ -403,11 +404,24 @@ static int device_records(const ModelSrc &src, int cm, int sx, int sy, int sz, i
 // Workgroups of the step kernel resident on one CU: the LDS rings (6 staged planes and 5
 // Dx planes of ty+8 rows) against 160 KB, and the waves per SIMD the kernel is built for
 // (step_waves: 4 for FP32, 6 for the other forms with 8- or 12-row tiles).
-static int resident_per_cu(int ty, int cm)
+// Workgroups of the step kernel resident on one CU: limited by their LDS and by the waves
+// per SIMD their registers allow (step_waves, pair_waves in the kernels).
+static int resident_per_cu(int ty, int cm, bool pair)
 {
+    if (pair) {  // use_pair(cm, ty, ..)
+        const long long lds = (2ll * (ty + 8) * 40 + 5ll * (ty + 8) * 32) * 8;
+        return std::max(1, (int)std::min<long long>((160ll << 10) / lds, 4 * 3 / (ty / 2)));  // 3 for ty 8
+    }
     const long long lds = (6ll * (ty + 8) * 80 + 5ll * (ty + 8) * 64) * 4;
     const int waves = (cm != CM_F32 && (ty == 8 || ty == 12)) ? 6 : 4;
     return std::max(1, (int)std::min<long long>((160ll << 10) / lds, 4 * waves / ty));
+}
+
+// The both-fields-per-lane kernel runs a form when it has one for the tile height and the
+// knob asks for it.
+static bool use_pair(int cm, int ty, bool twopass)
+{
+    return g_tiles.pair != 0 && has_pair_form(cm) && has_pair_tiles(ty) && !twopass;
 }
 
 // z chunk depth for `tiles` tile columns per plane over nzu planes: the chunk count that
@@ -416,7 +430,7 @@ static int resident_per_cu(int ty, int cm)
 // other forms (1536 tiles, three rounds of 512), the autotuner's choices.
 static int chunk_depth(long long tiles, int nzu, int ty, int cm, int ztmax)
 {
-    const long long slots = (long long)std::max(g_cus, 1) * resident_per_cu(ty, cm);
+    const long long slots = (long long)std::max(g_cus, 1) * resident_per_cu(ty, cm, use_pair(cm, ty, false));
     double best = 1e300;
     int bz = nzu;
     for (int nzc = 1; nzc <= 64; nzc++) {
@@ -463,7 +477,7 @@ static bool iso_spacing(float dx, float dy, float dz) { return dx == dy && dy ==
 
 static int launch_steps(const StepArgs &a, int ty, int cm, bool tp, hipStream_t s)
 {
-    if (launch_grid(ty, a, cm, tp, iso_spacing(D.dx, D.dy, D.dz), g_tiles.hints != 0, s) == 0)
+    if (launch_grid(ty, a, cm, tp, iso_spacing(D.dx, D.dy, D.dz), g_tiles.hints != 0, use_pair(cm, ty, tp), s) == 0)
         return FL_OK;
     return fail(FL_ERR_ARG, "unsupported tile rows %d / coefficient mode %d", ty, cm);
 }
@@ -633,7 +647,7 @@ static int take_events(hipEvent_t *e0, hipEvent_t *e1)
 struct GraphSlot {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    int K = 0, ty = 0, cm = 0, hints = 0, variant = 0;
+    int K = 0, ty = 0, cm = 0, hints = 0, variant = 0, pair = 0;
     StepArgs a0, a1;
     unsigned long long used = 0;
 };
@@ -959,6 +973,15 @@ extern "C" int fl_get_tile_strip(void)
     return base_args().sw;
 }
 
+extern "C" int fl_set_lane_pairs(int on)
+{
+    if (on != 0 && on != 1) return fail(FL_ERR_ARG, "lane pairs must be 0 or 1 (got %d)", on);
+    g_tiles.pair = on;
+    return FL_OK;
+}
+
+extern "C" int fl_get_lane_pairs(void) { return g_tiles.pair; }
+
 extern "C" int fl_get_tiles(int *ty, int *zt)
 {
     if (ty) *ty = g_tiles.ty;
@@ -1044,7 +1067,7 @@ static int step_graph(int K, F &&args, GraphSlot **out)
     const StepArgs a0 = args(0), a1 = args(1);
     for (auto &g : g_graphs)
         if (g.exec && g.K == K && g.ty == g_tiles.ty && g.cm == D.cm && g.hints == g_tiles.hints &&
-            g.variant == g_variant && memcmp(&g.a0, &a0, sizeof(StepArgs)) == 0 &&
+            g.variant == g_variant && g.pair == g_tiles.pair && memcmp(&g.a0, &a0, sizeof(StepArgs)) == 0 &&
             memcmp(&g.a1, &a1, sizeof(StepArgs)) == 0) {
             g.used = ++g_graph_tick;
             *out = &g;
@@ -1084,6 +1107,7 @@ static int step_graph(int K, F &&args, GraphSlot **out)
     g->cm = D.cm;
     g->hints = g_tiles.hints;
     g->variant = g_variant;
+    g->pair = g_tiles.pair;
     g->a0 = a0;
     g->a1 = a1;
     g->used = ++g_graph_tick;
@@ -1351,7 +1375,10 @@ static std::string tune_key(const char *kind, int sx, int sy, int sz, int cm)
     for (char *c = name; *c; c++)
         if (*c == ' ') *c = '_';
     char buf[400];
-    snprintf(buf, sizeof(buf), "%s|%s|%dx%dx%d|v%d|p%d", name, kind, sx, sy, sz, g_variant, cm);
+    // the lane layout (fl_set_lane_pairs) changes which tiles are fastest for the forms that
+    // have both: it is part of the key for them
+    const int lp = has_pair_form(cm) ? g_tiles.pair : 0;
+    snprintf(buf, sizeof(buf), "%s|%s|%dx%dx%d|v%d|p%d%s", name, kind, sx, sy, sz, g_variant, cm, lp ? "|lp" : "");
     return buf;
 }
 

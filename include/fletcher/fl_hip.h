@@ -83,6 +83,14 @@ int fl_get_stream_hints(void);
 int fl_set_tile_strip(int sw);
 int fl_get_tile_strip(void);
 
+/* Lane layout of the z-marching step for the 16-bit recompute forms (fp16r, q16r) on 8-row
+ * tiles: 1 (the default) runs both fields of a cell in one lane (workgroups of 4 waves, each
+ * wave two tile rows), 0 one field per lane.  Other forms and tile heights always run one
+ * field per lane.  Results are identical either way; single-grid and batched steps both
+ * follow it.  No reference counterpart (the reference has one thread per cell). */
+int fl_set_lane_pairs(int on);
+int fl_get_lane_pairs(void);
+
 /* Time every candidate (ty, zt), with and without stream hints, for
  * `steps` steps on the initialised grid and keep the fastest; fields are restored
  * afterwards.  When cache_path is not NULL the choice is looked up / appended there, keyed

@@ -408,8 +408,8 @@ static int device_records(const ModelSrc &src, int cm, int sx, int sy, int sz, i
 // per SIMD their registers allow (step_waves, pair_waves in the kernels).
 static int resident_per_cu(int ty, int cm, bool pair)
 {
-    if (pair) {  // use_pair(cm, ty, ..)
-        const long long lds = (2ll * (ty + 8) * 40 + 5ll * (ty + 8) * 32) * 8;
+    if (pair) {  // use_pair(cm, ty, ..): StepLds2
+        const long long lds = (2ll * (ty + 8) * 40 + 10ll * (ty + 8) * 32) * 8;
         return std::max(1, (int)std::min<long long>((160ll << 10) / lds, 4 * 3 / (ty / 2)));  // 3 for ty 8
     }
     const long long lds = (6ll * (ty + 8) * 80 + 5ll * (ty + 8) * 64) * 4;

@@ -21,11 +21,11 @@ int launch_batch_12(const BatchArgs &a, int grid, int cm, bool iso, bool pair, h
 int launch_batch_16(const BatchArgs &a, int grid, int cm, bool iso, bool pair, hipStream_t s);
 
 // Forms and tile heights that have the both-fields-per-lane kernel: 8-row tiles only.  At
-// 168 VGPRs a SIMD holds three of its waves; the 4-wave workgroups of 8-row tiles then fill
-// a CU with three workgroups, one wave of each per SIMD.  The 6- and 8-wave workgroups of
-// 12- and 16-row tiles leave a CU room for one workgroup (its waves are not spread evenly
+// up to 168 VGPRs a SIMD holds three of its waves; the 4-wave workgroups of 8-row tiles then
+// fill a CU with three workgroups, one wave of each per SIMD.  The 6- and 8-wave workgroups
+// of 12- and 16-row tiles left a CU room for one workgroup (its waves are not spread evenly
 // enough over the SIMDs for a second at 3 per SIMD) and ran 25-35% slower than the
-// one-field-per-lane kernel; 4-row tiles spill in the steady loop.
+// one-field-per-lane kernel; 4-row tiles (two halo row pairs a wave) spilled in the loop.
 constexpr bool has_pair_form(int cm) { return cm == CM_F16R || cm == CM_Q16R; }
 constexpr bool has_pair_tiles(int ty) { return ty == 8; }
 

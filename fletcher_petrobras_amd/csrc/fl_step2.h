@@ -2,27 +2,31 @@
 // (CM_F16R, CM_Q16R): the same z-marching as fl_step.h, the same expressions in the same
 // association order, so the results are bitwise those of tti_step.
 //
-// Why a second lane layout.  The recompute forms are bound by VALU issue and LDS bandwidth,
+// Why a second lane layout.  The recompute forms are bound by VALU issue and LDS traffic,
 // not by HBM.  With one field per lane (tti_step) both lanes of a cell decode the record
 // and form the six angular coefficients, and the pair exchanges records and H1/H2 through
-// DPP moves and per-lane selects: about 40% of the instructions per plane.  With one cell
-// per lane each is done once per cell, and the two fields' stencils share every LDS read
-// (pairs, 8 bytes a lane).
+// DPP moves and per-lane selects.  With one cell per lane each is done once per cell, and
+// the two fields' stencils share every LDS read (pairs, 8 bytes a lane): about 23% fewer
+// VALU instructions and 30% less LDS traffic per cell.
 //
 // Work decomposition.  A workgroup of TY/2 waves owns a 32 x TY tile of the xy plane and a
 // z chunk; wave w covers tile rows 2w (lanes 0-31) and 2w+1 (lanes 32-63), lane l the
 // column l % 32 of its row, both fields.
 //
 // On chip, per lane:
-//   registers  10-slot queues of c, Dx, Dy (pairs) over planes f-8 .. f, as tti_step;
+//   registers  10-slot queues of c and Dy (pairs) over planes f-8 .. f, as tti_step;
 //              5-slot queues of Dxx, Dyy: formed at the plane f they belong to, from the
 //              neighbours Dx and Dy read anyway, and used four planes later when f is the
 //              output plane (the staged plane is read once per step instead of twice);
 //   LDS        2 staged planes of (pc, qc) pairs (tile + 4-cell halo): plane f, read in the
-//              step, and plane f+1, written in it; 5 planes of Dx (tile rows + 4-row y
-//              halo), Dxy(z) = Der1_y(Dx(z)) read four steps after Dx(z) is written.
+//              step, and plane f+1, written in it; 10 planes of Dx (tile rows + 4-row y
+//              halo): Dxy(z) = Der1_y(Dx(z)) and Dxz(z) = Der1_z(Dx) (the own column of
+//              planes z-4 .. z+3; z+4 is the step's own) read from them.  Keeping Dx there
+//              instead of in a register queue frees the 20 VGPRs that keep the steady loop
+//              free of spills.
 // Every ring position is a compile-time constant in the ten unrolled steps of the steady
-// loop (2 and 5 divide 10), so no LDS address is formed at run time there.
+// loop (2 and 10 divide 10, and so does the 5 of the Dxx, Dyy queues), so no LDS address is
+// formed at run time there.
 #pragma once
 #include "fl_step.h"
 
@@ -134,10 +138,10 @@ __device__ __forceinline__ void decode_pair(const CoefRec<CM> &r, CoefPair &k)
 }
 
 template <int TY>
-struct StepLds2 {
+struct StepLds2 {  // 51 KB for 8-row tiles
     static constexpr int HX = 40, HY = TY + 8;
     F2 t[2][HY * HX];  // staged planes: (row, column) of the tile + halo
-    F2 x[5][HY * 32];  // Dx: (row, tile column), tile rows + y halo
+    F2 x[10][HY * 32];  // Dx: (row, tile column), tile rows + y halo
 };
 
 // One tile of one z chunk (the work of step_tile, fields paired per lane).
@@ -229,9 +233,9 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
         ohx[m] = (r - 2 * wv) * 32;
     }
 
-    F2 qc_[10], qx_[10], qy_[10], qa_[5], qb_[5];  // c, Dx, Dy; Dxx, Dyy
+    F2 qc_[10], qy_[10], qa_[5], qb_[5];  // c, Dy; Dxx, Dyy (Dx: the LDS ring)
 #pragma unroll
-    for (int k = 0; k < 10; k++) qc_[k] = qx_[k] = qy_[k] = F2{0.f, 0.f};
+    for (int k = 0; k < 10; k++) qc_[k] = qy_[k] = F2{0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 5; k++) qa_[k] = qb_[k] = F2{0.f, 0.f};
     CoefRec<CM> kp[2];
@@ -242,7 +246,7 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
     fetch(1);
     __syncthreads();
 
-    int r5 = 0;  // Dx slot of plane i (i % 5) in the generic steps
+    int r10 = 0;  // Dx slot of plane i (i % 10) in the generic steps
     // one input plane i (f = f0 + i); template arguments as in step_tile:
     //   PH  queue slot of plane i (static steps: i % 10), XW Dx (and Dxx, Dyy) of plane i
     //   0 never / 1 always / 2 if i < nz+4, OUT output plane z = f-4, GEN generic copy
@@ -258,8 +262,8 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
         const int f = f0 + i;
         const int z = f - 4;
         const F2 *const Tf = s.t[PAR];
-        F2 *const Xf = s.x[GEN ? r5 : PH % 5];
-        const F2 *const Xz = s.x[GEN ? (r5 == 4 ? 0 : r5 + 1) : (PH + 1) % 5];
+        F2 *const Xf = s.x[GEN ? r10 : PH];
+        const F2 *const Xz = s.x[GEN ? (r10 >= 4 ? r10 - 4 : r10 + 6) : (PH + 6) % 10];  // plane i-4
         const uint32_t so_o = (uint32_t)(i - 8) * pb8;
 
         CoefRec<CM> kr;
@@ -281,8 +285,8 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
         __builtin_amdgcn_sched_barrier(0);
         der12_pr<!ISO, XW != 0>(cp, HX, nc, a.dyinv, a.dyyinv, ny, nyy);
         __builtin_amdgcn_sched_barrier(0);
+        Xf[oxw + CX] = nx;  // every plane: Dxz reads it
         if (xw) {
-            Xf[oxw + CX] = nx;
 #pragma unroll
             for (int m = 0; m < NH; m++)
                 if (hrow[m]) Xf[oxw + ohx[m]] = der1_pr<!ISO>(Tf + ow + ohr[m] + 4, 1, a.dxinv);
@@ -292,7 +296,6 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
         fl_pin(ny);
         constexpr int ins = GEN ? 8 : PH, ins5 = GEN ? 4 : PH % 5;
         qc_[ins] = nc;
-        qx_[ins] = nx;
         qy_[ins] = ny;
         if constexpr (XW != 0) {
             fl_pin(nxx);
@@ -315,8 +318,12 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
 #pragma unroll
             for (int k = 0; k < 9; k++) {
                 zp[k] = qc_[(b + k) % 10].p; zq[k] = qc_[(b + k) % 10].q;
-                xp[k] = qx_[(b + k) % 10].p; xq[k] = qx_[(b + k) % 10].q;
                 yp[k] = qy_[(b + k) % 10].p; yq[k] = qy_[(b + k) % 10].q;
+                // Dx of plane i-8+k from its ring slot (the centre plane is not used), plane
+                // i from the register
+                F2 v = nx;
+                if (k < 8 && k != 4) v = s.x[GEN ? (r10 + 2 + k) % 10 : (PH + 2 + k) % 10][oxw + CX];
+                xp[k] = v.p; xq[k] = v.q;
             }
             const F2 dxx = qa_[b5], dyy = qb_[b5];
             const float dzzp = der2_q<!ISO>(zp, a.dzzinv), dzzq = der2_q<!ISO>(zq, a.dzzinv);
@@ -386,7 +393,6 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 qc_[k] = qc_[k + 1];
-                qx_[k] = qx_[k + 1];
                 qy_[k] = qy_[k + 1];
             }
 #pragma unroll
@@ -394,7 +400,7 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
                 qa_[k] = qa_[k + 1];
                 qb_[k] = qb_[k + 1];
             }
-            r5 = r5 == 4 ? 0 : r5 + 1;
+            r10 = r10 == 9 ? 0 : r10 + 1;
         }
         __syncthreads();
     };
@@ -434,7 +440,7 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
         qa_[2] = qa_[1]; qb_[2] = qb_[1];
         qa_[1] = qa_[0]; qb_[1] = qb_[0];
         qa_[0] = a4; qb_[0] = b4;
-        r5 = 3;  // (8 + 10 g) % 5
+        r10 = 8;  // (8 + 10 g) % 10
     }
     for (; i + 1 <= nz + 7; i += 2) {
         step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<0>{}, ic<0>{}, i, false);
@@ -443,9 +449,9 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
     if (i <= nz + 7) step(ic<0>{}, ic<2>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<0>{}, ic<0>{}, i, false);
 }
 
-// Waves per SIMD asked of the register allocator: 3 (168 VGPRs; 80 hold the queues), with
-// which the steady loop is free of spills; at 4 (128 VGPRs) it spills 48 values.  The 8-row
-// workgroups (4 waves, 31 KB of LDS) then run three to a CU.
+// Waves per SIMD asked of the register allocator: 3 (168 VGPRs; the steady loop uses 150
+// with no spills).  The 8-row workgroups (4 waves, 51 KB of LDS) then run three to a CU,
+// one wave of each per SIMD.
 template <int TY>
 constexpr int pair_waves()
 {

@@ -1,7 +1,8 @@
 """fl_run_steps_static: the multi-step run without any allocation (scratch in a fixed device
 array), checked through the C ABI against fl_run_steps on the same inputs: bit for bit, on
 two calls in a row, the second on a smaller grid, and with the source inside and outside
-the update domain."""
+the update domain; FP32 and the FP16-recompute form (whose 8-row tiles run the
+both-fields-per-lane kernel)."""
 import numpy as np
 import pytest
 
@@ -16,9 +17,9 @@ def _case(fl, n3, absorb, seed):
     return m, m[0].shape, fl.cfl_dt(m, 12.5, 0.4)
 
 
-def _run(fl, m, shape, dt, isrc, st, rec, static):
+def _run(fl, m, shape, dt, isrc, st, rec, static, prec):
     L = fl.lib()
-    d = fl.Driver(variant="fused", tiles=(8, 0))
+    d = fl.Driver(variant="fused", precision=prec, tiles=(8, 0))
     d.initialize(shape, D, dt, m)
     d.record_traces(rec, 3, st // 3)
     src = fl.sources(dt, st)
@@ -34,7 +35,8 @@ def _run(fl, m, shape, dt, isrc, st, rec, static):
     return out
 
 
-def test_static_run_equals_run_steps_twice(fl):
+@pytest.mark.parametrize("prec", ["fp32", "fp16r"])
+def test_static_run_equals_run_steps_twice(fl, prec):
     m1, s1, dt1 = _case(fl, (44, 40, 36), 6, 21)
     m2, s2, dt2 = _case(fl, (20, 18, 16), 4, 22)
     assert np.prod(s2) < np.prod(s1)
@@ -42,8 +44,9 @@ def test_static_run_equals_run_steps_twice(fl):
              (m2, s2, dt2, fl.index(s2, 14, 13, 10), 24, 9),    # smaller grid, fewer steps
              (m2, s2, dt2, fl.index(s2, 2, 13, 10), 18, 9)]     # source in the halo: never stored
     for m, s, dt, isrc, st, rec in calls:
-        a = _run(fl, m, s, dt, isrc, st, rec, static=False)
-        b = _run(fl, m, s, dt, isrc, st, rec, static=True)
+        print("static run: %s shape %s steps %d source %d" % (prec, s, st, isrc), flush=True)
+        a = _run(fl, m, s, dt, isrc, st, rec, static=False, prec=prec)
+        b = _run(fl, m, s, dt, isrc, st, rec, static=True, prec=prec)
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
         assert float(np.abs(a[0]).max()) > 0

@@ -248,9 +248,12 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
 
     int r10 = 0;  // Dx slot of plane i (i % 10) in the generic steps
     // one input plane i (f = f0 + i); template arguments as in step_tile:
-    //   PH  queue slot of plane i (static steps: i % 10), XW Dx (and Dxx, Dyy) of plane i
-    //   0 never / 1 always / 2 if i < nz+4, OUT output plane z = f-4, GEN generic copy
-    //   (queues shifted afterwards), PF prefetch bits, PAR i % 2, PS prefetch set
+    //   PH  queue slot of plane i (static steps: i % 10)
+    //   XW  the y-halo rows of Dx of plane i: 0 never / 1 always / 2 if i < nz+4 (f is an
+    //       output plane); Dxx, Dyy of plane i whenever XW != 0.  The tile rows of Dx go to
+    //       the ring on every plane (Dxz reads them)
+    //   OUT output plane z = f-4, GEN generic copy (queues shifted afterwards), PF prefetch
+    //   bits, PAR i % 2 (staged slot), PS prefetch set
     auto step = [&](auto PHc, auto XWc, auto OUTc, auto GENc, auto PFc, auto PARc, auto PSc, const int i,
                     const bool pf_next) {
         constexpr int PH = decltype(PHc)::value;

@@ -11,7 +11,7 @@
 // the previous one (own cell) and overwrites the previous one; the roles then swap.
 // The coefficients are records of one cell each, at the same pitched cell index.
 //
-// Kernels here (the z-marching step kernel is in fl_step.h):
+// Kernels here (the z-marching step kernels are in fl_step.h, fl_step2.h and fl_step_batch.h):
 //   direct_dercross  the reference's direct 64-point DerCross form
 //       (original/derivatives.h:59-68), one thread per (x,y) column.
 //   der1_pass        pass 1 of the two-pass form (der1der1/OpenMP/openmp_propagate.c:24-76).

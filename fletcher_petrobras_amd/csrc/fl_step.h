@@ -31,6 +31,10 @@
 //
 // TWOPASS takes Dx, Dy from the four first-derivative volumes of der1_pass
 // (der1der1/OpenMP/openmp_propagate.c:24-76) instead of forming them on chip.
+//
+// fl_step2.h holds the same step with both fields of a cell in one lane (the 16-bit
+// recompute forms on 8-row tiles), fl_step_batch.h the batched step over a (shot, tile)
+// work list built on either.
 #pragma once
 #include <type_traits>
 #include <utility>

@@ -165,7 +165,8 @@ extern "C" int fl_bytes_per_cell(void)
 // Host precompute into the device coefficient records: per cell, its coef_count(cm)
 // values (FP32 or the 16-bit forms, or the recompute inputs) in the slot order
 // rec_order32 / rec_order16 / rec_order_r32 / rec_order_r16 (record of
-// coef_count * coef_elem bytes); z slabs of at most 64 MiB per array on the host.
+// coef_count * coef_elem bytes; FP32 in its two arrays, slots 0-3, 5-8 and slots 4, 9);
+// z slabs of at most 64 MiB per array on the host.
 // ebase: the CM_Q16R exponent base (fl_recompute_q16_base over the whole grid).
 static int upload_coefs(const float *vpz, const float *vsv, const float *eps, const float *del,
                         const float *phi, const float *theta, float dt, int cm, void *dst, int sx,
@@ -192,9 +193,21 @@ static int upload_coefs(const float *vpz, const float *vsv, const float *eps, co
                 fl_pack_recompute32(m, vpz + off, vsv + off, eps + off, del + off, phi + off, theta + off, c);
             const float *co[10];
             for (int k = 0; k < nk; k++) co[k] = c[cm == CM_F32 ? rec_order32(k) : rec_order_r32(k)];
-            fl_interleave32(m, nk, co, r32.data());
-            HIPCHK(hipMemcpy2D((char *)dst + (size_t)z0 * sy * px * rec, px * rec, r32.data(), sx * rec,
-                               sx * rec, (size_t)sy * (z1 - z0), hipMemcpyHostToDevice));
+            if (cm == CM_F32) {  // slots 0-3, 5-8 at 32 B a cell, then slots 4, 9 at 8 B a cell
+                const float *c1[8] = {co[0], co[1], co[2], co[3], co[5], co[6], co[7], co[8]};
+                const float *c2[2] = {co[4], co[9]};
+                const long long np = (long long)px * sy * sz;
+                fl_interleave32(m, 8, c1, r32.data());
+                HIPCHK(hipMemcpy2D((char *)dst + (size_t)z0 * sy * px * 32, px * 32, r32.data(), sx * 32, sx * 32,
+                                   (size_t)sy * (z1 - z0), hipMemcpyHostToDevice));
+                fl_interleave32(m, 2, c2, r32.data());
+                HIPCHK(hipMemcpy2D((char *)dst + fp32_part2(np) + (size_t)z0 * sy * px * 8, px * 8, r32.data(),
+                                   sx * 8, sx * 8, (size_t)sy * (z1 - z0), hipMemcpyHostToDevice));
+            } else {
+                fl_interleave32(m, nk, co, r32.data());
+                HIPCHK(hipMemcpy2D((char *)dst + (size_t)z0 * sy * px * rec, px * rec, r32.data(), sx * rec,
+                                   sx * rec, (size_t)sy * (z1 - z0), hipMemcpyHostToDevice));
+            }
         } else {
             h16.resize((size_t)nk * m);
             r16.resize((size_t)nk * m);
@@ -270,11 +283,16 @@ __global__ __launch_bounds__(256) void precompute_records(ModelVols v, long long
         const bool pad = (int)(c % px) >= sx;
         const float vp = pad ? 0.f : v.m[0][c], vs = pad ? 0.f : v.m[1][c], e = pad ? 0.f : v.m[2][c];
         const float d = pad ? 0.f : v.m[3][c], ph = pad ? 0.f : v.m[4][c], th = pad ? 0.f : v.m[5][c];
-        if constexpr (CM == CM_F32) {
+        if constexpr (CM == CM_F32) {  // the two arrays: slots 0-3, 5-8 and slots 4, 9
             float k[10];
             fl_cell_f32(vp, vs, e, d, ph, th, k);
-            float *r = (float *)rec + c * 10;
-            for (int s = 0; s < 10; s++) r[s] = pad ? 0.f : k[rec_order32(s)];
+            float *r1 = (float *)rec + c * 8;
+            float *r2 = (float *)((char *)rec + fp32_part2(np)) + c * 2;
+            for (int s = 0; s < 10; s++) {
+                const float v = pad ? 0.f : k[rec_order32(s)];
+                if (s == 4 || s == 9) r2[s == 9] = v;
+                else r1[s < 4 ? s : s - 1] = v;
+            }
         } else if constexpr (CM == CM_F32R) {
             float k[6];
             fl_cell_recompute32(vp, vs, e, d, ph, th, k);
@@ -597,6 +615,7 @@ static int launch_step(StepArgs a, int it_host, hipStream_t s)
         d.prev = a.prev;
         d.cur = a.cur;
         d.coef = (const float *)D.coef;
+        d.np = D.np;
         d.px = D.px;
         d.dxxinv = a.dxxinv; d.dyyinv = a.dyyinv; d.dzzinv = a.dzzinv;
         d.dxyinv = 1.0f / (D.dx * D.dy);
@@ -1017,6 +1036,18 @@ extern "C" int fl_get_records(void *host)
     if (!host) return fail(FL_ERR_ARG, "host buffer must not be NULL");
     HIPCHK(hipStreamSynchronize(g_stream));
     HIPCHK(hipMemcpy(host, D.coef, (size_t)fl_record_bytes(), hipMemcpyDeviceToHost));
+    if (D.cm == CM_F32) {  // the two device arrays back to one 10-slot record a cell
+        const long long np = D.np;
+        std::vector<float> dev((const float *)host, (const float *)host + np * 10);
+        float *out = (float *)host;
+        const float *a1 = dev.data(), *a2 = dev.data() + np * 8;
+        for (long long c = 0; c < np; c++) {
+            for (int s = 0; s < 4; s++) out[c * 10 + s] = a1[c * 8 + s];
+            out[c * 10 + 4] = a2[c * 2];
+            for (int s = 5; s < 9; s++) out[c * 10 + s] = a1[c * 8 + s - 1];
+            out[c * 10 + 9] = a2[c * 2 + 1];
+        }
+    }
     return FL_OK;
 }
 

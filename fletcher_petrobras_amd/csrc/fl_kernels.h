@@ -66,7 +66,11 @@ __host__ __device__ constexpr int rec_bytes(int cm)
 // coefficient REC_ORDER[s].  The step kernel has the two lanes of a cell (p and q) load
 // one half of the record each and broadcast it to the pair:
 //   FP32 (40 B):        p lane {0 1 2 3 6}, q lane {4 5 7 8 9}; the fifth value is the
-//                       lane's own v2px / v2pn.
+//                       lane's own v2px / v2pn.  On the device the record is held in two
+//                       arrays of the pitched cells (fp32_part2): 32 bytes {0 1 2 3 4 5 7 8}
+//                       at 32 c, then, after all cells, 8 bytes {6 9} at 32 np + 8 c.  A
+//                       lane's two loads (16 and 4 bytes) then each read whole lines
+//                       across the wave, which lets them stream non-temporally.
 //   BF16 / FP16 (20 B): p lane {0 1 2 3}, q lane {4 5 7 8}, both {6 9}.
 // The recompute inputs are v2px, v2pz, v2sz, v2pn, theta, phi (0..5), stored as
 //   FP16 and block floating-point recompute (12 B): words (v2px, v2pn), (v2pz, v2sz),
@@ -77,6 +81,9 @@ __host__ __device__ constexpr int rec_order32(int s)
     constexpr int o[10] = {0, 1, 2, 3, 6, 4, 5, 7, 8, 9};
     return o[s];
 }
+// Byte offset of the 8-byte part of the FP32 records of a grid of np pitched cells.
+__host__ __device__ constexpr long long fp32_part2(long long np) { return np * 32; }
+
 __host__ __device__ constexpr int rec_order16(int s)
 {
     constexpr int o[10] = {0, 1, 2, 3, 4, 5, 7, 8, 6, 9};
@@ -301,7 +308,8 @@ __device__ __forceinline__ void tile_of(int t, int ntx, int nty, int sw, int &tx
 struct DirectArgs {
     float *prev;
     const float *cur;
-    const float *coef;    // FP32 records of 10 per cell
+    const float *coef;    // FP32 records (the two arrays of rec_order32's note)
+    long long np;         // pitched cells of the grid
     float dxxinv, dyyinv, dzzinv, dxyinv, dxzinv, dyzinv, dt;
     int sx, sy, sz, bord, px, zt;
     int ntx, nty, nblk;   // tiles of 64 x DX_TY cells along x, y; z chunks of zt planes
@@ -448,14 +456,16 @@ void direct_dercross(const DirectArgs a)
             const PQ yz = dercross_runs([&](int u, int v) { return at(pl[4 + v], u * DX_HX); }, a.dyzinv);
             const PQ xz = dercross_runs([&](int u, int v) { return at(pl[4 + v], u); }, a.dxzinv);
             const long long cell = (long long)z * pxy + (long long)y * a.px + x;
-            const float2 *rec = (const float2 *)(a.coef + cell * 10);
+            const float4 *r1 = (const float4 *)(a.coef + cell * 8);  // slots 0-3, 5-8
+            const float2 r2 = ((const float2 *)((const char *)a.coef + fp32_part2(a.np)))[cell];  // 4, 9
+            const float4 u = r1[0], w = r1[1];
             Coef k;
-#pragma unroll
-            for (int w = 0; w < 5; w++) {
-                const float2 v = rec[w];
-                k.c[rec_order32(2 * w)] = v.x;
-                k.c[rec_order32(2 * w + 1)] = v.y;
-            }
+            k.c[rec_order32(0)] = u.x; k.c[rec_order32(1)] = u.y;
+            k.c[rec_order32(2)] = u.z; k.c[rec_order32(3)] = u.w;
+            k.c[rec_order32(4)] = r2.x;
+            k.c[rec_order32(5)] = w.x; k.c[rec_order32(6)] = w.y;
+            k.c[rec_order32(7)] = w.z; k.c[rec_order32(8)] = w.w;
+            k.c[rec_order32(9)] = r2.y;
             float2 *pv = (float2 *)a.prev + cell;
             float2 o = *pv;
             const PQ cc = {pl[4][0].x, pl[4][0].y};

@@ -68,15 +68,16 @@ constexpr int prefetch_steps()
 }
 
 // Streaming hints (PCPOL 2, off 0): the previous level, read and written once per step, and
-// the coefficient records that one load reads whole (the recompute forms) are loaded and
-// stored non-temporal.  Records read by two loads (FP32, BF16, FP16) keep the default
-// policy: the hint on either load made those steps slower.  Whether the hints pay depends
+// the coefficient records whose loads read whole lines across a wave (the recompute forms,
+// and FP32 in its two arrays) are loaded and stored non-temporal.  Records whose lines two
+// loads share (BF16, FP16) keep the default policy: the hint on either load made those
+// steps slower.  Whether the hints pay depends
 // on the grid (faster on a 384^3 array, slower on a 1096x1096x840 one), so the host
 // selects them per grid.
 template <int CM, int PCPOL>
 constexpr int coef_cpol()
 {
-    return (PCPOL && (CM == CM_F16R || CM == CM_F32R || CM == CM_Q16R)) ? 2 : 0;
+    return (PCPOL && (CM == CM_F32 || CM == CM_F16R || CM == CM_F32R || CM == CM_Q16R)) ? 2 : 0;
 }
 
 // Lane byte offset beyond every buffer range used here (spans are < 2^31 bytes).
@@ -103,8 +104,9 @@ struct CoefRec {
     uint32_t w[NW];
 };
 
-// Lane byte offsets of a cell's record: the own half (voff) and, for the 16-bit forms,
-// the shared last word (voff2).
+// Lane byte offsets of a cell's record: the own half (voff) and, for the 16-bit forms, the
+// shared last word (voff2); FP32: the own 16 bytes in the first array and the own 4 bytes
+// in the second (rec_order32).
 struct RecOff {
     uint32_t voff, voff2;
 };
@@ -113,8 +115,8 @@ __device__ __forceinline__ RecOff rec_offsets(uint32_t col, int fld, bool active
 {
     RecOff o;
     if constexpr (CM == CM_F32) {
-        o.voff = active ? col * 40u + (uint32_t)fld * 20u : FL_OOB;
-        o.voff2 = 0;
+        o.voff = active ? col * 32u + (uint32_t)fld * 16u : FL_OOB;
+        o.voff2 = active ? col * 8u + (uint32_t)fld * 4u : FL_OOB;
     } else if constexpr (CM == CM_F16R || CM == CM_Q16R) {
         o.voff = active ? col * 12u : FL_OOB;
         o.voff2 = 0;
@@ -128,20 +130,56 @@ __device__ __forceinline__ RecOff rec_offsets(uint32_t col, int fld, bool active
     return o;
 }
 
+// Loads of one record: the recompute forms whole (one 12- or 24-byte load), BF16 and FP16
+// the lane's half and the shared word.
 template <int CM, int PCPOL>
 __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, const RecOff &o, CoefRec<CM> &c)
 {
-    if constexpr (CM == CM_F32) {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, o.voff, 0, 0);
-        c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
-        c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff, 16, 0);
-    } else if constexpr (CM == CM_F16R || CM == CM_F32R || CM == CM_Q16R) {
+    static_assert(CM != CM_F32, "FP32 records: load_next_rec");
+    if constexpr (CM == CM_F16R || CM == CM_F32R || CM == CM_Q16R) {
         const auto a = __builtin_amdgcn_raw_buffer_load_b96(r, o.voff, 0, coef_cpol<CM, PCPOL>());
         c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2];
     } else {
         const auto a = __builtin_amdgcn_raw_buffer_load_b64(r, o.voff, 0, 0);
         c.w[0] = a[0]; c.w[1] = a[1];
         c.w[2] = __builtin_amdgcn_raw_buffer_load_b32(r, o.voff2, 0, 0);
+    }
+}
+
+// The records of the output planes of a chunk, one plane after the other: p the records (FP32:
+// its first array), p2 the second FP32 array.
+struct RecStream {
+    const char *p, *p2;
+};
+template <int CM>
+__device__ __forceinline__ RecStream rec_stream(const void *coef, long long np, long long cell0)
+{
+    RecStream r;
+    if constexpr (CM == CM_F32) {
+        r.p = (const char *)coef + cell0 * 32;
+        r.p2 = (const char *)coef + fp32_part2(np) + cell0 * 8;
+    } else {
+        r.p = (const char *)coef + cell0 * rec_bytes(CM);
+        r.p2 = nullptr;
+    }
+    return r;
+}
+
+// Loads the next plane's record of the lane's cell and advances the stream (pxy cells a plane).
+template <int CM, int PCPOL>
+__device__ __forceinline__ void load_next_rec(RecStream &rs, uint32_t pxy, const RecOff &o, CoefRec<CM> &c)
+{
+    if constexpr (CM == CM_F32) {
+        constexpr int cp = coef_cpol<CM, PCPOL>();
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(span_rsrc(rs.p, pxy * 32u), o.voff, 0, cp);
+        c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
+        c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(span_rsrc(rs.p2, pxy * 8u), o.voff2, 0, cp);
+        rs.p += pxy * 32u;
+        rs.p2 += pxy * 8u;
+    } else {
+        const uint32_t pbr = pxy * (uint32_t)rec_bytes(CM);
+        load_rec<CM, PCPOL>(span_rsrc(rs.p, pbr), o, c);
+        rs.p += pbr;
     }
 }
 
@@ -280,7 +318,6 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     constexpr int NS = RW * HY;                // floats per staged plane (a multiple of 64)
     constexpr int NLD = (NS + NT - 1) / NT;    // staging loads per lane
     constexpr int XW2 = 2 * TX;                // floats per Dx row
-    constexpr int REC = rec_bytes(CM);
     constexpr int NH = (8 + TY - 1) / TY;      // y-halo rows of Dx per wave (at most)
     static_assert(NS % 64 == 0, "staged plane must be whole waves");
     const int x0 = a.xorg + tix * TX, y0 = a.bord + tiy * TY;
@@ -300,7 +337,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     const float dts = ISO ? a.dxxinv * a.dt : a.dt;  // scale of rhs in the FP32 leapfrog
     const float rq = ISO ? a.dxxinv * a.vq : a.vq;   // and in the block floating-point form
     const uint32_t pxy = (uint32_t)a.px * (uint32_t)a.sy;  // cells per pitched plane
-    const uint32_t pb8 = pxy * 8u, pbr = pxy * (uint32_t)REC;
+    const uint32_t pb8 = pxy * 8u;
     const uint32_t col = (uint32_t)(y * a.px + x);
     const uint32_t vo8 = active ? col * 8u + fld * 4u : FL_OOB;  // own (cell, field) float
     const RecOff vor = rec_offsets<CM>(col, fld, active);
@@ -308,7 +345,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     const long long zin = (long long)f0 * pxy, zout = (long long)zlo * pxy;
     const __amdgpu_buffer_rsrc_t r_cur = span_rsrc(a.cur + 2 * zin, (uint32_t)(nz + 8) * pb8);
     const __amdgpu_buffer_rsrc_t r_prev = span_rsrc(a.prev + 2 * zout, (uint32_t)nz * pb8);
-    const char *cpl = (const char *)a.coef + zout * REC;  // records of the next output plane
+    RecStream recs = rec_stream<CM>(a.coef, (long long)pxy * a.sz, zout);  // next output plane's records
 
     // source cell of this step (added to both fields of the output) and receiver plane
     int src_z = -1;
@@ -426,8 +463,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             kr = kp[PS];
             vprev = ppv[PS];
         } else if constexpr (OUT) {
-            load_rec<CM, PCPOL>(span_rsrc(cpl, pbr), vor, kr);
-            cpl += pbr;
+            load_next_rec<CM, PCPOL>(recs, pxy, vor, kr);
             vprev = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_prev, vo8, so_o, PCPOL));
         }
         const bool xw = (XW == 1) || (XW == 2 && i < nz + 4);
@@ -536,8 +572,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         if constexpr ((PF & 2) != 0 || (PF & 4) != 0) {
             if ((PF & 2) || pf_next) {
                 constexpr int NS2 = (PS + PD) % NSETS;
-                load_rec<CM, PCPOL>(span_rsrc(cpl, pbr), vor, kp[NS2]);
-                cpl += pbr;
+                load_next_rec<CM, PCPOL>(recs, pxy, vor, kp[NS2]);
                 ppv[NS2] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_prev, vo8, (uint32_t)(i + PD - 8) * pb8, PCPOL));
             }
         }

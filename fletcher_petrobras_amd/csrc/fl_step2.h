@@ -155,8 +155,7 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
     constexpr int NLD = (NSP + NT - 1) / NT;  // staging loads per lane
     constexpr int REC = rec_bytes(CM);
     constexpr int NH = (4 + W - 1) / W;       // y-halo row pairs of Dx per wave (at most)
-    constexpr int PD = prefetch_steps<CM>() > 0 ? prefetch_steps<CM>() : 1;
-    static_assert(PD <= 2, "two prefetch register sets");
+    constexpr int PD = 2;  // records and previous level two steps ahead, in two register sets
     const int x0 = a.xorg + tix * TX, y0 = a.bord + tiy * TY;
     const int zlo = a.bord + zc * a.zt;
     const int zhi = min(zlo + a.zt, a.sz - a.bord);
@@ -408,7 +407,7 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
         __syncthreads();
     };
 
-    constexpr int PFB = prefetch_steps<CM>() ? 4 : 0;
+    constexpr int PFB = 4;
     const bool grp0 = 8 + 9 < nz + 4;
     auto pro = [&](auto Jc, auto XWc) {
         constexpr int j = decltype(Jc)::value;
@@ -429,7 +428,7 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
         unroll_phases(
             [&](auto P) {
                 constexpr int p = decltype(P)::value;
-                constexpr int pf = prefetch_steps<CM>() ? (1 | (p + PD <= 9 ? 2 : 4)) : 0;
+                constexpr int pf = 1 | (p + PD <= 9 ? 2 : 4);
                 step(ic<(8 + p) % 10>{}, ic<1>{}, ic<1>{}, ic<0>{}, ic<pf>{}, ic<p & 1>{}, ic<p % 2>{}, i + p, more);
             },
             std::make_integer_sequence<int, 10>{});

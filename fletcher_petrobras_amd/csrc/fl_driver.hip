@@ -1476,7 +1476,7 @@ extern "C" int fl_autotune(int steps, const char *cache_path)
             for (int s = 0; s < steps && rc == FL_OK; s++) {
                 StepArgs b = base_args();
                 b.trace = nullptr;
-                rc = launch_step(b, 0, g_stream);
+                rc = launch_step(b, s & 1, g_stream);  // both tile orders (step_order)
                 swap_levels();
             }
             HIPCHK(hipEventRecord(e1, g_stream));

@@ -132,6 +132,7 @@ struct StepCtl {
     float *trace;   // plane to write this step, or nullptr
     float src_add;
     bool do_src;
+    int it;         // step index
 };
 
 // Wave-uniform copies (readfirstlane) of values the compiler computes or loads in vector
@@ -183,6 +184,7 @@ __device__ __forceinline__ StepCtl step_ctl(const StepArgs &a)
     }
     c.trace = uni(c.trace);
     c.src_add = uni(c.src_add);
+    c.it = uni(it);
     return c;
 }
 
@@ -283,6 +285,13 @@ __device__ __forceinline__ int xcd_remap(int b, int g)
 {
     return (b & 7) * (g >> 3) + (b >> 3);
 }
+
+// Odd steps take the logical blocks in reverse (t -> nblk-1-t): a step then starts on the
+// tiles the step before ended on, whose level planes were the last to pass through the
+// memory-side cache.  Each tile's result does not depend on the order.  At 344^3 the step
+// runs 1.2% (FP32, 12-row tiles) and 1.6% (FP16 recompute, 8-row tiles, both fields per
+// lane) faster than with one fixed order.
+__device__ __forceinline__ int step_order(int t, int n, int it) { return (it & 1) ? n - 1 - t : t; }
 
 // Tile (tx, ty, tz) of logical block t: z chunks outermost; inside a chunk, strips of sw
 // tile columns (the last one narrower) walked row by row.  The tiles an XCD holds at once

@@ -656,8 +656,9 @@ void tti_step(const StepArgs a)
     const StepCtl ctl = step_ctl(a);
     if (a.run && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0)
         a.run[1 - a.par] = a.run[a.par] + 1;
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
-    if (t >= a.nblk) return;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    if (lb >= a.nblk) return;
+    const int t = step_order(lb, a.nblk, ctl.it);
     int tx, ty, tz;
     tile_of(t, a.ntx, a.nty, a.sw, tx, ty, tz);
     step_tile<TY, CM, TWOPASS, ISO, HINTS ? 2 : 0>(a, ctl, uni(tx), uni(ty), uni(tz), s);

@@ -128,9 +128,9 @@ void tti_step_batch(const BatchArgs b, const ShotDev *__restrict__ shots, const 
             step_tile<TY, CM, false, ISO, batch_cpol<CM>()>(a, ctl, tix, uni(rest % a.nty), uni(rest / a.nty), s);
     };
     if constexpr (!PERSIST) {
-        const int t = xcd_remap(blockIdx.x, gridDim.x);
-        if (t >= b.nitems) return;
-        item(t);
+        const int lb = xcd_remap(blockIdx.x, gridDim.x);
+        if (lb >= b.nitems) return;
+        item(step_order(lb, b.nitems, it));
     } else {
         __shared__ int tk;
         const int x = blockIdx.x & 7, per = gridDim.x >> 3;  // the grid is whole rounds of 8

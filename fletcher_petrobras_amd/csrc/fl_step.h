@@ -146,21 +146,29 @@ __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t r, const RecOff 
     }
 }
 
-// The records of the output planes of a chunk, one plane after the other: p the records (FP32:
-// its first array), p2 the second FP32 array.
+// The records of the output planes of a chunk, one plane after the other, upwards or downwards
+// in z: p the records (FP32: its first array), p2 the second FP32 array, d1 and d2 their
+// signed steps from one plane to the next.
 struct RecStream {
     const char *p, *p2;
+    long long d1, d2;
 };
 template <int CM>
-__device__ __forceinline__ RecStream rec_stream(const void *coef, long long np, long long cell0)
+__device__ __forceinline__ RecStream rec_stream(const void *coef, long long np, long long cell0, uint32_t pxy,
+                                                bool down)
 {
     RecStream r;
+    const long long sg = down ? -1 : 1;
     if constexpr (CM == CM_F32) {
         r.p = (const char *)coef + cell0 * 32;
         r.p2 = (const char *)coef + fp32_part2(np) + cell0 * 8;
+        r.d1 = sg * (long long)pxy * 32;
+        r.d2 = sg * (long long)pxy * 8;
     } else {
         r.p = (const char *)coef + cell0 * rec_bytes(CM);
         r.p2 = nullptr;
+        r.d1 = sg * (long long)pxy * rec_bytes(CM);
+        r.d2 = 0;
     }
     return r;
 }
@@ -174,12 +182,12 @@ __device__ __forceinline__ void load_next_rec(RecStream &rs, uint32_t pxy, const
         const auto a = __builtin_amdgcn_raw_buffer_load_b128(span_rsrc(rs.p, pxy * 32u), o.voff, 0, cp);
         c.w[0] = a[0]; c.w[1] = a[1]; c.w[2] = a[2]; c.w[3] = a[3];
         c.w[4] = __builtin_amdgcn_raw_buffer_load_b32(span_rsrc(rs.p2, pxy * 8u), o.voff2, 0, cp);
-        rs.p += pxy * 32u;
-        rs.p2 += pxy * 8u;
+        rs.p += rs.d1;
+        rs.p2 += rs.d2;
     } else {
         const uint32_t pbr = pxy * (uint32_t)rec_bytes(CM);
         load_rec<CM, PCPOL>(span_rsrc(rs.p, pbr), o, c);
-        rs.p += pbr;
+        rs.p += rs.d1;
     }
 }
 
@@ -309,9 +317,14 @@ __device__ __forceinline__ void unroll_phases(F &&fn, std::integer_sequence<int,
 // ISO (dx = dy = dz = h): the derivatives stay unscaled sums and 1/h^2 multiplies the
 // right-hand side once, in the leapfrog (the same expression up to rounding: every term
 // of H1, H2 carries 1/h^2).
+// down: the chunk is marched from its top plane to its bottom one.  Step i then takes input
+// plane zhi+3-i instead of zlo-4+i, the z queues hold the planes in reverse order, and the
+// z second derivative, a sum of symmetric pairs, comes out bit for bit the same; each z
+// first derivative comes out exactly negated (every operation of the sum is odd in its
+// inputs) and its sign is restored by a negative dz scale (or a multiply by -1 when ISO).
 template <int TY, int CM, bool TWOPASS, bool ISO, int PCPOL>
 __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl, int tix, int tiy, int zc,
-                                          StepLds<TY> &s)
+                                          bool down, StepLds<TY> &s)
 {
     constexpr int TX = 32, HX = 40, HY = TY + 8, NT = 64 * TY;
     constexpr int RW = 2 * HX;                 // floats per staged row (pairs)
@@ -345,7 +358,14 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     const long long zin = (long long)f0 * pxy, zout = (long long)zlo * pxy;
     const __amdgpu_buffer_rsrc_t r_cur = span_rsrc(a.cur + 2 * zin, (uint32_t)(nz + 8) * pb8);
     const __amdgpu_buffer_rsrc_t r_prev = span_rsrc(a.prev + 2 * zout, (uint32_t)nz * pb8);
-    RecStream recs = rec_stream<CM>(a.coef, (long long)pxy * a.sz, zout);  // next output plane's records
+    // input plane of step i (from f0) and its output plane (from zlo), in either direction
+    const int pin0 = down ? nz + 7 : 0, pout0 = down ? nz + 7 : -8, pdir = down ? -1 : 1;
+    auto pin = [&](int i) { return (uint32_t)(pin0 + pdir * i); };
+    auto pout = [&](int i) { return pout0 + pdir * i; };
+    const float dzs = down ? -a.dzinv : a.dzinv, zsg = down ? -1.0f : 1.0f;
+    // next output plane's records
+    RecStream recs = rec_stream<CM>(a.coef, (long long)pxy * a.sz, down ? zout + (long long)(nz - 1) * pxy : zout,
+                                    pxy, down);
 
     // source cell of this step (added to both fields of the output) and receiver plane
     int src_z = -1;
@@ -372,9 +392,9 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     }
     constexpr int SA = FL_STAGE_AHEAD;
     float pre[SA][NLD];  // raw loads of plane j in set j % SA
-    auto fetch = [&](auto Sc, int i) {  // loads of input plane f0 + i into set Sc
+    auto fetch = [&](auto Sc, int i) {  // loads of the input plane of step i into set Sc
         constexpr int S = decltype(Sc)::value;
-        const uint32_t so = (uint32_t)i * pb8;
+        const uint32_t so = pin(i) * pb8;
 #pragma unroll
         for (int k = 0; k < NLD; k++)
             if (hk[k]) pre[S][k] = __builtin_bit_cast(float, bload32(r_cur, hoff[k], so));
@@ -420,15 +440,15 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
     CoefRec<CM> kp[NSETS];
     float ppv[NSETS] = {};
 
-    // prologue: plane f0 into ring slot 0, the next SA planes in flight
+    // prologue: the first input plane into ring slot 0, the next SA planes in flight
     fetch(ic<0>{}, 0);
     stage(ic<0>{}, s.t[0]);
     fetch(ic<1 % SA>{}, 1);
     if constexpr (SA == 2) fetch(ic<0>{}, 2);
     __syncthreads();
 
-    int r6 = 0, r5 = 0;  // LDS ring slots of plane f0 + i: i % 6, i % 5
-    // one input plane i (f = f0 + i):
+    int r6 = 0, r5 = 0;  // LDS ring slots of the input plane of step i: i % 6, i % 5
+    // one input plane (step i; plane f0 + i upwards, zhi + 3 - i downwards):
     //   PH   queue slot of plane i (static steps: i % 10)
     //   XW   Dx halo planes 0 never / 1 always / 2 if i < nz+4 (f is an output plane)
     //   OUT  output plane z = f-4
@@ -447,14 +467,13 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         constexpr int SET = decltype(PARc)::value;
         constexpr int PS = decltype(PSc)::value;
         constexpr int PD = prefetch_steps<CM>() > 0 ? prefetch_steps<CM>() : 1;
-        const int f = f0 + i;
-        const int z = f - 4;
+        const int z = zlo + pout(i);  // output plane
         const int r6n = r6 == 5 ? 0 : r6 + 1, r5n = r5 == 4 ? 0 : r5 + 1;
         float *const Tf = s.t[r6];
         float *const Tz = s.t[r6 >= 4 ? r6 - 4 : r6 + 2];
         float *const Xf = s.x[r5];
         float *const Xz = s.x[r5n];
-        const uint32_t so_o = (uint32_t)(i - 8) * pb8;
+        const uint32_t so_o = (uint32_t)pout(i) * pb8;
 
         // streams of the output plane, kept raw until used
         CoefRec<CM> kr;
@@ -469,7 +488,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
         const bool xw = (XW == 1) || (XW == 2 && i < nz + 4);
         float nx, ny, gxh[NH];
         if constexpr (TWOPASS) {
-            const uint32_t so = (uint32_t)i * pb8;
+            const uint32_t so = pin(i) * pb8;
             nx = __builtin_bit_cast(float, bload32(r_dx, vo8, so));
             ny = __builtin_bit_cast(float, bload32(r_dy, vo8, so));
             if (xw) {
@@ -530,8 +549,12 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
                 qzy[k] = qy_[(b + k) % 10];
             }
             const float dzz = der2_q<!ISO>(qz, a.dzzinv);
-            const float dxz = der1_q<!ISO>(qzx, a.dzinv);
-            const float dyz = der1_q<!ISO>(qzy, a.dzinv);
+            float dxz = der1_q<!ISO>(qzx, dzs);
+            float dyz = der1_q<!ISO>(qzy, dzs);
+            if constexpr (ISO) {
+                dxz *= zsg;
+                dyz *= zsg;
+            }
             Coef kc;
             decode_rec<CM>(kr, fld, kc);
             // H1, H2 of the own field, association order of original/sample.h:46-53
@@ -573,7 +596,8 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, const StepCtl &ctl,
             if ((PF & 2) || pf_next) {
                 constexpr int NS2 = (PS + PD) % NSETS;
                 load_next_rec<CM, PCPOL>(recs, pxy, vor, kp[NS2]);
-                ppv[NS2] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r_prev, vo8, (uint32_t)(i + PD - 8) * pb8, PCPOL));
+                ppv[NS2] = __builtin_bit_cast(
+                    float, __builtin_amdgcn_raw_buffer_load_b32(r_prev, vo8, (uint32_t)pout(i + PD) * pb8, PCPOL));
             }
         }
         if constexpr (GEN) {
@@ -661,7 +685,7 @@ void tti_step(const StepArgs a)
     const int t = step_order(lb, a.nblk, ctl.it);
     int tx, ty, tz;
     tile_of(t, a.ntx, a.nty, a.sw, tx, ty, tz);
-    step_tile<TY, CM, TWOPASS, ISO, HINTS ? 2 : 0>(a, ctl, uni(tx), uni(ty), uni(tz), s);
+    step_tile<TY, CM, TWOPASS, ISO, HINTS ? 2 : 0>(a, ctl, uni(tx), uni(ty), uni(tz), step_down(ctl.it), s);
 }
 
 }  // namespace fl

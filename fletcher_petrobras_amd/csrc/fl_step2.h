@@ -99,14 +99,15 @@ __device__ __forceinline__ F2 der1_pr(const F2 *c, int s, float dinv)
 
 // The decoded record of one cell: the six angular coefficients (precomp.h order 0-5), the
 // velocity factors and, for CM_Q16R, 2^e.  Each value is the product tti_step forms in one
-// of the two lanes (decode_rec), with the same operands.
+// of the two lanes (decode_rec), with the same operands.  zs = -1 negates c[4] and c[5]
+// exactly (the coefficients of the z first derivatives, for a downward march).
 struct CoefPair {
     float c[6];
     float own_p, own_q, vpz, vsz, e2;
 };
 
 template <int CM>
-__device__ __forceinline__ void decode_pair(const CoefRec<CM> &r, CoefPair &k)
+__device__ __forceinline__ void decode_pair(const CoefRec<CM> &r, CoefPair &k, float zs)
 {
     static_assert(CM == CM_F16R || CM == CM_Q16R, "recompute records only");
     if constexpr (CM == CM_F16R) {  // words (v2px, v2pn), (v2pz, v2sz), (theta, phi)
@@ -127,7 +128,7 @@ __device__ __forceinline__ void decode_pair(const CoefRec<CM> &r, CoefPair &k)
     const float tp = (float)(r.w[2] >> 16) * (1.0f / 65536.0f);      // phi
     const float st = __builtin_amdgcn_sinf(tt), ct = __builtin_amdgcn_cosf(tt);
     const float sp = __builtin_amdgcn_sinf(tp), cp = __builtin_amdgcn_cosf(tp);
-    const float st2 = st * st, ct2 = ct * ct, s2t = (st + st) * ct;  // sin^2, cos^2, sin 2 theta
+    const float st2 = st * st, ct2 = ct * ct, s2t = (st + st) * (ct * zs);  // sin^2, cos^2, +-sin 2 theta
     const float sp2 = sp * sp, cp2 = cp * cp, s2p = (sp + sp) * cp;  // the same of phi
     k.c[0] = st2 * cp2;  // sin^2 theta cos^2 phi
     k.c[1] = st2 * sp2;  // sin^2 theta sin^2 phi
@@ -144,10 +145,11 @@ struct StepLds2 {  // 51 KB for 8-row tiles
     F2 x[10][HY * 32];  // Dx: (row, tile column), tile rows + y halo
 };
 
-// One tile of one z chunk (the work of step_tile, fields paired per lane).
+// One tile of one z chunk (the work of step_tile, fields paired per lane), upwards or, with
+// down, from the top plane down as step_tile does it.
 template <int TY, int CM, bool ISO, int PCPOL>
 __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl, int tix, int tiy, int zc,
-                                           StepLds2<TY> &s)
+                                           bool down, StepLds2<TY> &s)
 {
     static_assert(TY % 2 == 0, "two tile rows per wave");
     constexpr int TX = 32, HX = 40, HY = TY + 8, W = TY / 2, NT = 64 * W;
@@ -177,7 +179,15 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
     const long long zin = (long long)f0 * pxy, zout = (long long)zlo * pxy;
     const __amdgpu_buffer_rsrc_t r_cur = span_rsrc(a.cur + 2 * zin, (uint32_t)(nz + 8) * pb8);
     const __amdgpu_buffer_rsrc_t r_prev = span_rsrc(a.prev + 2 * zout, (uint32_t)nz * pb8);
-    const char *cpl = (const char *)a.coef + zout * REC;
+    // input plane of step i (from f0) and its output plane (from zlo), in either direction
+    const int pin0 = down ? nz + 7 : 0, pout0 = down ? nz + 7 : -8, pdir = down ? -1 : 1;
+    auto pin = [&](int i) { return (uint32_t)(pin0 + pdir * i); };
+    auto pout = [&](int i) { return pout0 + pdir * i; };
+    const float dzs = down ? -a.dzinv : a.dzinv;
+    const float zsc = (ISO && down) ? -1.0f : 1.0f;  // sign of c[4], c[5] (decode_pair)
+    // records of the next output plane, and the step to the one after
+    const char *cpl = (const char *)a.coef + (down ? zout + (long long)(nz - 1) * pxy : zout) * REC;
+    const long long cst = down ? -(long long)pbr : (long long)pbr;
 
     int src_z = -1;
     uint32_t src_col = 0;
@@ -202,8 +212,8 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
         hoff[k] = ok ? (uint32_t)(gy * a.px + gx) * 8u : FL_OOB;
     }
     F2 pre[NLD];
-    auto fetch = [&](int i) {
-        const uint32_t so = (uint32_t)i * pb8;
+    auto fetch = [&](int i) {  // the input plane of step i
+        const uint32_t so = pin(i) * pb8;
 #pragma unroll
         for (int k = 0; k < NLD; k++)
             if (hk[k]) pre[k] = f2_of(__builtin_amdgcn_raw_buffer_load_b64(r_cur, hoff[k], so, 0));
@@ -246,7 +256,7 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
     __syncthreads();
 
     int r10 = 0;  // Dx slot of plane i (i % 10) in the generic steps
-    // one input plane i (f = f0 + i); template arguments as in step_tile:
+    // one input plane (step i); template arguments as in step_tile:
     //   PH  queue slot of plane i (static steps: i % 10)
     //   XW  the y-halo rows of Dx of plane i: 0 never / 1 always / 2 if i < nz+4 (f is an
     //       output plane); Dxx, Dyy of plane i whenever XW != 0.  The tile rows of Dx go to
@@ -261,12 +271,11 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
         constexpr int PF = decltype(PFc)::value;
         constexpr int PAR = decltype(PARc)::value;
         constexpr int PS = decltype(PSc)::value;
-        const int f = f0 + i;
-        const int z = f - 4;
+        const int z = zlo + pout(i);  // output plane
         const F2 *const Tf = s.t[PAR];
         F2 *const Xf = s.x[GEN ? r10 : PH];
         const F2 *const Xz = s.x[GEN ? (r10 >= 4 ? r10 - 4 : r10 + 6) : (PH + 6) % 10];  // plane i-4
-        const uint32_t so_o = (uint32_t)(i - 8) * pb8;
+        const uint32_t so_o = (uint32_t)pout(i) * pb8;
 
         CoefRec<CM> kr;
         F2 vprev = {0.f, 0.f};
@@ -275,7 +284,7 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
             vprev = ppv[PS];
         } else if constexpr (OUT) {
             load_rec<CM, PCPOL>(span_rsrc(cpl, pbr), vor, kr);
-            cpl += pbr;
+            cpl += cst;
             vprev = f2_of(__builtin_amdgcn_raw_buffer_load_b64(r_prev, vo8, so_o, PCPOL));
         }
         const bool xw = (XW == 1) || (XW == 2 && i < nz + 4);
@@ -329,10 +338,12 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
             }
             const F2 dxx = qa_[b5], dyy = qb_[b5];
             const float dzzp = der2_q<!ISO>(zp, a.dzzinv), dzzq = der2_q<!ISO>(zq, a.dzzinv);
-            const float dxzp = der1_q<!ISO>(xp, a.dzinv), dxzq = der1_q<!ISO>(xq, a.dzinv);
-            const float dyzp = der1_q<!ISO>(yp, a.dzinv), dyzq = der1_q<!ISO>(yq, a.dzinv);
+            // the z first derivatives of a downward march come out negated: dzs < 0 restores
+            // their sign, or (ISO, unscaled sums) the negated c[4], c[5]
+            const float dxzp = der1_q<!ISO>(xp, dzs), dxzq = der1_q<!ISO>(xq, dzs);
+            const float dyzp = der1_q<!ISO>(yp, dzs), dyzq = der1_q<!ISO>(yq, dzs);
             CoefPair kc;
-            decode_pair<CM>(kr, kc);
+            decode_pair<CM>(kr, kc, zsc);
             // H1, H2 of both fields, association order of original/sample.h:46-53
             float h1p = kc.c[0] * dxx.p;
             h1p = __builtin_fmaf(kc.c[1], dyy.p, h1p);
@@ -386,9 +397,9 @@ __device__ __forceinline__ void step_tile2(const StepArgs &a, const StepCtl &ctl
             if ((PF & 2) || pf_next) {
                 constexpr int NS2 = (PS + PD) % 2;
                 load_rec<CM, PCPOL>(span_rsrc(cpl, pbr), vor, kp[NS2]);
-                cpl += pbr;
+                cpl += cst;
                 ppv[NS2] = f2_of(
-                    __builtin_amdgcn_raw_buffer_load_b64(r_prev, vo8, (uint32_t)(i + PD - 8) * pb8, PCPOL));
+                    __builtin_amdgcn_raw_buffer_load_b64(r_prev, vo8, (uint32_t)pout(i + PD) * pb8, PCPOL));
             }
         }
         if constexpr (GEN) {
@@ -477,7 +488,7 @@ void tti_step2(const StepArgs a)
     const int t = step_order(lb, a.nblk, ctl.it);
     int tx, ty, tz;
     tile_of(t, a.ntx, a.nty, a.sw, tx, ty, tz);
-    step_tile2<TY, CM, ISO, HINTS ? 2 : 0>(a, ctl, uni(tx), uni(ty), uni(tz), s);
+    step_tile2<TY, CM, ISO, HINTS ? 2 : 0>(a, ctl, uni(tx), uni(ty), uni(tz), step_down(ctl.it), s);
 }
 
 }  // namespace fl

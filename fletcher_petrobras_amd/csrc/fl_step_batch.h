@@ -123,9 +123,10 @@ void tti_step_batch(const BatchArgs b, const ShotDev *__restrict__ shots, const 
         const int tix = uni(l % a.ntx);
         const int rest = uni(l / a.ntx);
         if constexpr (PAIR)
-            step_tile2<TY, CM, ISO, 0>(a, ctl, tix, uni(rest % a.nty), uni(rest / a.nty), s);
+            step_tile2<TY, CM, ISO, 0>(a, ctl, tix, uni(rest % a.nty), uni(rest / a.nty), false, s);
         else
-            step_tile<TY, CM, false, ISO, batch_cpol<CM>()>(a, ctl, tix, uni(rest % a.nty), uni(rest / a.nty), s);
+            step_tile<TY, CM, false, ISO, batch_cpol<CM>()>(a, ctl, tix, uni(rest % a.nty), uni(rest / a.nty), false,
+                                                            s);
     };
     if constexpr (!PERSIST) {
         const int lb = xcd_remap(blockIdx.x, gridDim.x);

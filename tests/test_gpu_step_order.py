@@ -2,9 +2,10 @@
 even steps go forwards and upwards (csrc/fl_kernels.h step_order, step_down).  One step from
 the same levels must give the same bits at an even and at an odd step index: the z second
 derivative is a sum of symmetric pairs, and the z first derivatives come out exactly negated
-and are scaled back exactly.  Cases: FP32 with isotropic and anisotropic spacing, the two-pass
-variant, FP16 recompute with both fields per lane and with one, q16r and BF16, ragged and
-whole z chunks, per-step calls and captured runs."""
+and are scaled back exactly.  The receiver plane each step writes is compared too.  Cases:
+FP32 with isotropic and anisotropic spacing, the two-pass variant, FP16 recompute with both
+fields per lane and with one, q16r and BF16, ragged and whole z chunks, per-step calls and
+captured runs."""
 import numpy as np
 import pytest
 
@@ -25,7 +26,7 @@ def _driver(fl, variant, prec, tiles, d3):
     d = fl.Driver(variant=variant, precision=prec, tiles=tiles)
     d.initialize(shape, d3, dt, m)
     d.step_loop(9, isrc)  # a wavefield that reaches every chunk
-    return d, isrc
+    return d, isrc, shape[0] // 2 + 2
 
 
 def _one_step(d, state, it, graph, isrc):
@@ -51,15 +52,20 @@ def _one_step(d, state, it, graph, isrc):
 ])
 def test_odd_step_equals_even_step(fl, variant, prec, tiles, d3, graph):
     print("step order: %s %s tiles %s spacing %s graph %s" % (variant, prec, tiles, d3, graph), flush=True)
-    d, isrc = _driver(fl, variant, prec, tiles, d3)
+    d, isrc, rec = _driver(fl, variant, prec, tiles, d3)
     try:
         state = {k: d.field(k) for k in LEVELS}
         assert np.abs(state["pc"]).max() > 0
+        d.record_traces(rec, 1, 12)  # step it writes slot it - 1
         even = _one_step(d, state, 10, graph, isrc)
         odd = _one_step(d, state, 11, graph, isrc)
         even2 = _one_step(d, state, 12, graph, isrc)
+        tr = d.traces()
     finally:
         d.finalize()
+    assert np.abs(tr[9]).max() > 0
+    assert np.array_equal(tr[10].view(np.uint32), tr[9].view(np.uint32))
+    assert np.array_equal(tr[11], tr[9])
     changed = False
     for k in LEVELS:
         assert np.array_equal(even[k], even2[k]), k

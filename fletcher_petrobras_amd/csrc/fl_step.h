@@ -2,8 +2,10 @@
 //
 // Work decomposition.  A workgroup of TY waves owns a 32 x TY tile of the xy plane and a
 // chunk [zlo, zhi) of z; it streams the input planes f = zlo-4 .. zhi+3 of pc, qc through
-// LDS once.  A wave covers 32 columns of one tile row for BOTH fields: lane 2c carries p
-// and lane 2c+1 q of column c, so each lane keeps only its own field's state.
+// LDS once (on odd steps from zhi+3 down to zlo-4: the planes below are then those above,
+// and "f-8 .. f" below reads "f+8 .. f").  A wave covers 32 columns of one tile row for
+// BOTH fields: lane 2c carries p and lane 2c+1 q of column c, so each lane keeps only its
+// own field's state.
 //
 // On chip, per lane (own field, own column):
 //   registers  9-deep z queues of c = pc|qc, Dx and Dy over planes f-8 .. f (Dzz, Dxz, Dyz
